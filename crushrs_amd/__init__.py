@@ -1,0 +1,274 @@
+"""MI355X crash-solver core: Python mirror of the crushrs solver API.
+
+The reference's interface is `crushrs::run(&Model) -> Results`
+(src/solver.rs:190) plus the scenario builders of src/vehicle.rs and
+src/headon.rs.  Here the same calls go through the C ABI of
+libcrushrs_amd.so (include/crushrs.h, include/crushrs_scenario.h); the
+solve itself runs on the GPU.  Nothing in this package computes on the CPU
+beyond model construction.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from pathlib import Path
+from typing import Optional
+
+import numpy as np
+
+from . import _abi
+from ._abi import CrushError, check, load
+
+MPH = 0.44704  # src/lib.rs:28
+GROUND_CLEARANCE = 0.15  # src/vehicle.rs:499
+TUNED_ELEMENT_SIZE = 0.3  # src/vehicle.rs:647
+PULSE_END = 0.15  # src/vehicle.rs:1104
+
+VEHICLE_DIR = Path(__file__).resolve().parent / "vehicles"
+
+
+def vehicle(name: str) -> _abi.Vehicle:
+    """A preset (neon, silverado, expedition, navigator, neon-pulse, ...)
+    or a vehicle TOML path (src/bin/crushrs.rs:246-260)."""
+    v = _abi.Vehicle()
+    check(load().crush_vehicle_load(str(name).encode(), C.byref(v)))
+    return v
+
+
+def vehicle_file(stem: str) -> _abi.Vehicle:
+    """One of the tuned vehicle TOMLs shipped with the package
+    (neon_1996, silverado_2007, expedition_1999, navigator_1999)."""
+    return vehicle(str(VEHICLE_DIR / f"{stem}.toml"))
+
+
+def renamed(v: _abi.Vehicle, name: str) -> _abi.Vehicle:
+    out = _abi.Vehicle()
+    C.pointer(out)[0] = v
+    out.name = name.encode()
+    return out
+
+
+def side_profile(v: _abi.Vehicle) -> _abi.Vehicle:
+    out = _abi.Vehicle()
+    check(load().crush_vehicle_side_profile(C.byref(v), C.byref(out)))
+    return out
+
+
+def rear_profile(v: _abi.Vehicle) -> _abi.Vehicle:
+    out = _abi.Vehicle()
+    check(load().crush_vehicle_rear_profile(C.byref(v), C.byref(out)))
+    return out
+
+
+def default_settings() -> _abi.Settings:
+    """Settings::default (src/model.rs:134-149)."""
+    s = _abi.Settings()
+    s.end_time = 0.15
+    s.dt_scale = 0.85
+    s.time_step = 0.0
+    s.hourglass = 0.1
+    s.velocity_damping = 1.0
+    s.mass_scaling = 0.25
+    s.adaptive_check_steps = 5
+    s.separation_steps = 0
+    return s
+
+
+class Model:
+    """An owned, flattened model (the reference's `Model`)."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+        self._lib = load()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.crush_model_free(h)
+            self._h = None
+
+    # ---- builders
+    @classmethod
+    def crash(cls, truck_mph: float = 30.0, car_mph: float = 30.0) -> "Model":
+        """`crushrs crash truck_mph car_mph` (src/bin/crushrs.rs:403-432)."""
+        h = C.c_void_p()
+        check(load().crush_scenario_crash(truck_mph, car_mph, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def headon(cls, a: _abi.Vehicle, b: _abi.Vehicle, speed_a: float, speed_b: float, offset: float = 0.0,
+               clearance: Optional[float] = GROUND_CLEARANCE, element_size: float = TUNED_ELEMENT_SIZE,
+               end_time: float = PULSE_END) -> "Model":
+        """HeadOn::build (src/headon.rs:40-66); speeds in m/s."""
+        o = _abi.HeadOnOptions(offset, -1.0 if clearance is None else clearance, element_size, end_time)
+        h = C.c_void_p()
+        check(load().crush_scenario_headon(C.byref(a), C.byref(b), speed_a, speed_b, C.byref(o), C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def tbone(cls, striking: _abi.Vehicle, struck: _abi.Vehicle, speed_a: float, speed_b: float,
+              offset: float = 0.0, clearance: Optional[float] = GROUND_CLEARANCE,
+              element_size: float = TUNED_ELEMENT_SIZE) -> "Model":
+        """`crushrs tbone` generalised (src/bin/crushrs.rs:550-573)."""
+        h = C.c_void_p()
+        check(load().crush_scenario_tbone(C.byref(striking), C.byref(struck), speed_a, speed_b, offset,
+                                          -1.0 if clearance is None else clearance, element_size, C.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def barrier(cls, v: _abi.Vehicle, element_size: float = TUNED_ELEMENT_SIZE, speed: float = 35 * MPH,
+                end_time: float = PULSE_END) -> "Model":
+        """barrier_model (src/vehicle.rs:951-973)."""
+        h = C.c_void_p()
+        check(load().crush_scenario_barrier(C.byref(v), element_size, speed, end_time, C.byref(h)))
+        return cls(h)
+
+    # ---- access
+    @property
+    def view(self) -> _abi.ModelView:
+        return self._lib.crush_model_view(self._h).contents
+
+    @property
+    def ptr(self):
+        return self._lib.crush_model_view(self._h)
+
+    @property
+    def settings(self) -> _abi.Settings:
+        s = _abi.Settings()
+        C.pointer(s)[0] = self.view.settings
+        return s
+
+    @settings.setter
+    def settings(self, s: _abi.Settings) -> None:
+        check(self._lib.crush_model_set_settings(self._h, C.byref(s)))
+
+    def set(self, **kw) -> "Model":
+        s = self.settings
+        for k, v in kw.items():
+            setattr(s, k, v)
+        self.settings = s
+        return self
+
+    @property
+    def n_nodes(self) -> int:
+        return self.view.n_nodes
+
+    @property
+    def n_hexes(self) -> int:
+        return self.view.n_hexes
+
+    def nodes(self) -> np.ndarray:
+        v = self.view
+        return np.ctypeslib.as_array(v.nodes, shape=(v.n_nodes, 3)).copy()
+
+    def hexes(self) -> np.ndarray:
+        v = self.view
+        return np.ctypeslib.as_array(v.hexes, shape=(v.n_hexes, 8)).copy()
+
+    def initial_velocity(self) -> np.ndarray:
+        v = self.view
+        return np.ctypeslib.as_array(v.initial_velocity, shape=(v.n_nodes, 3)).copy()
+
+    def node_set(self, name: str, face_set: bool = False) -> np.ndarray:
+        p = _abi.i32p()
+        n = C.c_int32()
+        check(self._lib.crush_model_node_set(self._h, name.encode(), int(face_set), C.byref(p), C.byref(n)))
+        return np.ctypeslib.as_array(p, shape=(n.value,)).copy() if n.value else np.zeros(0, np.int32)
+
+    def part_names(self) -> list[str]:
+        buf = C.create_string_buffer(1 << 16)
+        check(self._lib.crush_model_part_names(self._h, buf, len(buf)))
+        return buf.value.decode().split("\n")
+
+    def contact_node_count(self) -> int:
+        n = C.c_int32()
+        check(self._lib.crush_contact_node_count(self.ptr, C.byref(n)))
+        return n.value
+
+
+@dataclass
+class GroupResult:
+    mass: float
+    delta_v: np.ndarray
+    crush_end: float
+    crush_max: float
+
+
+@dataclass
+class Results:
+    """src/solver.rs:63-80, plus the per-vehicle outputs."""
+    time: float
+    steps: int
+    eroded_elements: int
+    wall_ms: float
+    device_ms: float
+    added_mass_max: float
+    separated: bool
+    displacement: np.ndarray
+    velocity: np.ndarray
+    masses: np.ndarray
+    groups: list = field(default_factory=list)
+
+    def mean_velocity(self, nodes) -> tuple[float, np.ndarray]:
+        """Results::mean_velocity / Model::mean_velocity (src/model.rs:281-295)."""
+        nodes = np.asarray(nodes)
+        m = self.masses[nodes]
+        mt = m.sum()
+        if mt <= 0:
+            return 0.0, np.zeros(3)
+        return float(mt), (m[:, None] * self.velocity[nodes]).sum(0) / mt
+
+
+def alloc_results(n_nodes: int, n_groups: int):
+    u = np.zeros((n_nodes, 3))
+    v = np.zeros((n_nodes, 3))
+    m = np.zeros(n_nodes)
+    g = (_abi.GroupResult * max(n_groups, 1))()
+    r = _abi.Results()
+    r.displacement = u.ctypes.data_as(_abi.f64p)
+    r.velocity = v.ctypes.data_as(_abi.f64p)
+    r.masses = m.ctypes.data_as(_abi.f64p)
+    r.groups = C.cast(g, C.POINTER(_abi.GroupResult))
+    return r, u, v, m, g
+
+
+def to_results(r, u, v, m, g) -> Results:
+    groups = [GroupResult(g[i].mass, np.array(g[i].delta_v[:]), g[i].crush_end, g[i].crush_max)
+              for i in range(r.n_groups)]
+    return Results(r.time, int(r.steps), int(r.eroded_elements), r.wall_ms, r.device_ms, r.added_mass_max,
+                   bool(r.separated), u, v, m, groups)
+
+
+def run(model: Model) -> Results:
+    """crushrs::run on the GPU (src/solver.rs:190-381)."""
+    lib = load()
+    r, u, v, m, g = alloc_results(model.n_nodes, model.view.n_groups)
+    check(lib.crush_run(model.ptr, C.byref(r)))
+    return to_results(r, u, v, m, g)
+
+
+def debug_run_steps(model: Model, n_steps: int):
+    """Per-step taps of the first n_steps on the GPU: f_int, f_ext, hit,
+    eroded, dt (see crush_debug_run_steps)."""
+    lib = load()
+    n3 = 3 * model.n_nodes
+    k = model.contact_node_count()
+    fi = np.zeros((n_steps, n3))
+    fx = np.zeros((n_steps, n3))
+    hit = np.zeros((n_steps, max(k, 1)), np.int32)
+    er = np.zeros((n_steps, model.n_hexes), np.uint8)
+    dt = np.zeros(n_steps)
+    done = C.c_int32()
+    check(lib.crush_debug_run_steps(model.ptr, n_steps, fi.ctypes.data_as(_abi.f64p), fx.ctypes.data_as(_abi.f64p),
+                                    hit.ctypes.data_as(_abi.i32p), er.ctypes.data_as(_abi.u8p),
+                                    dt.ctypes.data_as(_abi.f64p), C.byref(done)))
+    s = done.value
+    return fi[:s], fx[:s], hit[:s, :k], er[:s], dt[:s]
+
+
+def device_count() -> int:
+    return int(load().crush_device_count())
+
+
+__all__ = ["MPH", "Model", "Results", "run", "vehicle", "vehicle_file", "side_profile", "rear_profile",
+           "default_settings", "debug_run_steps", "device_count", "CrushError", "renamed"]

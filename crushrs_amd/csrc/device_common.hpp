@@ -1,0 +1,528 @@
+// Device-side building blocks shared by the one-solve kernels and the
+// batched kernels: the f32 element force (honeycomb rate form and Hencky
+// elastic), the f64 node-to-face contact test and the f64 characteristic
+// length.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "prep.hpp"
+
+namespace crush {
+
+// Per-solve control block in device memory.
+struct StepCtl {
+    double t, dt, dt_cfl, end_time;
+    double contact_dt, dt_floor, dt_scale, damping;
+    double added_total_max;
+    long long step;
+    unsigned long long eroded_total;
+    int done, error, scale_now, had_pair;
+    int quiet, separated, pair_hits, chk;
+    int sep_steps, adaptive, reserved0, reserved1;
+    double crush_max[8];
+};
+
+// Face geometry record width (doubles) in the SoA face array:
+// x0..x3 (12), normal (3), centre (3), inflated lo (3), hi (3),
+// candidate lo (3), candidate hi (3), size (1).
+constexpr int FG = 31;
+constexpr int FG_N = 12, FG_C = 15, FG_LO = 18, FG_HI = 21, FG_CLO = 24, FG_CHI = 27, FG_SZ = 30;
+
+// Contact constants: src/contact.rs:75 (MAX_ACCEL), :87 (SOFT), :34 (edge
+// tolerance of the triangle test).
+constexpr double C_MAX_ACCEL = 1.0e5;
+constexpr double C_SOFT = 0.1;
+constexpr double C_TRI_TOL = 0.1;
+
+// ------------------------------------------------------------ f32 3x3
+
+__device__ __forceinline__ float det3f(const float* m) {
+    return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
+           m[2] * (m[3] * m[7] - m[4] * m[6]);
+}
+
+// Cofactor inverse with a given determinant.
+__device__ __forceinline__ void inv3f(const float* m, float det, float* r) {
+    float id = 1.0f / det;
+    r[0] = (m[4] * m[8] - m[5] * m[7]) * id;
+    r[1] = (m[2] * m[7] - m[1] * m[8]) * id;
+    r[2] = (m[1] * m[5] - m[2] * m[4]) * id;
+    r[3] = (m[5] * m[6] - m[3] * m[8]) * id;
+    r[4] = (m[0] * m[8] - m[2] * m[6]) * id;
+    r[5] = (m[2] * m[3] - m[0] * m[5]) * id;
+    r[6] = (m[3] * m[7] - m[4] * m[6]) * id;
+    r[7] = (m[1] * m[6] - m[0] * m[7]) * id;
+    r[8] = (m[0] * m[4] - m[1] * m[3]) * id;
+}
+
+__device__ __forceinline__ void mul3f(const float* a, const float* b, float* r) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r[3 * i + j] = a[3 * i] * b[j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
+}
+
+// ------------------------------------------------- element force (f32)
+
+// Honeycomb rate-form stress update (src/material.rs:472-535, the f32
+// lane form of src/kernel_simd.rs:336-443): F the deformation gradient,
+// st[25] = F_prev^-1 (9), R (9), sigma_hat (6), compaction; updates st and
+// returns the Kirchhoff stress tau (row-major).
+__device__ __forceinline__ void honeycomb_update(const MatParams& mp, const float* F, float* st, float* tau) {
+    float finc[9], mid[9], midinv[9], g[9], dinc[9];
+    mul3f(F, st, finc);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        float id = (k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f;
+        dinc[k] = finc[k] - id;
+        mid[k] = 0.5f * dinc[k] + id;
+    }
+    inv3f(mid, det3f(mid), midinv);
+    mul3f(dinc, midinv, g);
+    float d[9], w[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            d[3 * i + j] = 0.5f * (g[3 * i + j] + g[3 * j + i]);
+            w[3 * i + j] = 0.5f * (g[3 * i + j] - g[3 * j + i]);
+        }
+    // Cayley rotation increment I + (W + W^2/2) / (1 + |omega|^2 / 4)
+    float omega2 = w[1] * w[1] + w[2] * w[2] + w[5] * w[5];
+    float inv_den = 1.0f / (1.0f + 0.25f * omega2);
+    float ww[9], dr[9], r[9];
+    mul3f(w, w, ww);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        float id = (k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f;
+        dr[k] = (w[k] + 0.5f * ww[k]) * inv_den + id;
+    }
+    mul3f(dr, st + 9, r);
+    // strain increment in the corotational frame: R^T d R
+    float rtd[9], de[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) rtd[3 * i + j] = r[i] * d[j] + r[3 + i] * d[3 + j] + r[6 + i] * d[6 + j];
+    mul3f(rtd, r, de);
+    const float e = mp.youngs;
+    float s[6];
+    s[0] = st[18] + e * de[0];
+    s[1] = st[19] + e * de[4];
+    s[2] = st[20] + e * de[8];
+    s[3] = st[21] + e * de[1];
+    s[4] = st[22] + e * de[5];
+    s[5] = st[23] + e * de[2];
+    const float c = st[24];
+    // knot scan (unused knots sit at +inf)
+    float kc = mp.knot_c[0], ks = mp.knot_s[0], kh = mp.knot_h[0];
+#pragma unroll
+    for (int i = 1; i < 8; ++i) {
+        bool m = c >= mp.knot_c[i];
+        kc = m ? mp.knot_c[i] : kc;
+        ks = m ? mp.knot_s[i] : ks;
+        kh = m ? mp.knot_h[i] : kh;
+    }
+    const float hard = kh;
+    const float syc = ks + hard * (c - kc);
+    const float tk[3] = {1.0f, mp.trans, mp.trans};
+    // Normal caps with implicit hardening: three passes of the active-set
+    // solve dc = sum_A(-s_k - t_k syc) / (E + H sum_A t_k).
+    float dc = 0.0f;
+#pragma unroll
+    for (int it = 0; it < 3; ++it) {
+        float sy = syc + hard * dc;
+        float sum = 0.0f, ta = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            bool over = -s[k] > tk[k] * sy;
+            sum += over ? (-s[k] - tk[k] * syc) : 0.0f;
+            ta += over ? tk[k] : 0.0f;
+        }
+        dc = ta > 0.0f ? sum / (e + ta * hard) : 0.0f;
+    }
+    const float sy = fmaxf(syc + hard * dc, 0.0f);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) s[k] = fminf(fmaxf(s[k], -tk[k] * sy), tk[k] * sy);
+    const float ty = 0.5f * sy;
+#pragma unroll
+    for (int k = 3; k < 6; ++k) s[k] = fminf(fmaxf(s[k], -ty), ty);
+    // tau = J R sigma_hat R^T
+    const float sh[9] = {s[0], s[3], s[5], s[3], s[1], s[4], s[5], s[4], s[2]};
+    float rs[9];
+    mul3f(r, sh, rs);
+    const float J = det3f(F);
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            tau[3 * i + j] = J * (rs[3 * i] * r[3 * j] + rs[3 * i + 1] * r[3 * j + 1] + rs[3 * i + 2] * r[3 * j + 2]);
+    // commit
+    inv3f(F, J, st);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) st[9 + k] = r[k];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) st[18 + k] = s[k];
+    st[24] = c + dc;
+}
+
+// Symmetric 3x3 eigen-decomposition (vals ascending irrelevant here):
+// trigonometric root for the well-separated eigenvalue, its eigenvector
+// from the largest row cross product, one exact Jacobi rotation in the
+// orthogonal plane and Rayleigh-quotient eigenvalues (the method of
+// src/material.rs:570-659).  Works on any scale; vecs column-major
+// (vecs[3*c + i] = component i of eigenvector c).
+__device__ __forceinline__ void sym_eigen3f(const float* a, float* vals, float* vecs) {
+    float a00 = a[0], a01 = a[1], a02 = a[2], a11 = a[4], a12 = a[5], a22 = a[8];
+    float mx = fmaxf(fmaxf(fmaxf(fabsf(a00), fabsf(a01)), fmaxf(fabsf(a02), fabsf(a11))), fmaxf(fabsf(a12), fabsf(a22)));
+    if (mx == 0.0f) {
+        vals[0] = vals[1] = vals[2] = 0.0f;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) vecs[k] = (k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f;
+        return;
+    }
+    const float inv = 1.0f / mx;
+    a00 *= inv; a01 *= inv; a02 *= inv; a11 *= inv; a12 *= inv; a22 *= inv;
+    const float off = a01 * a01 + a02 * a02 + a12 * a12;
+    if (off == 0.0f) {
+        vals[0] = a00 * mx; vals[1] = a11 * mx; vals[2] = a22 * mx;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) vecs[k] = (k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f;
+        return;
+    }
+    const float q = (a00 + a11 + a22) * (1.0f / 3.0f);
+    const float b00 = a00 - q, b11 = a11 - q, b22 = a22 - q;
+    const float p = sqrtf((b00 * b00 + b11 * b11 + b22 * b22 + 2.0f * off) * (1.0f / 6.0f));
+    const float c00 = b11 * b22 - a12 * a12;
+    const float c01 = a01 * b22 - a12 * a02;
+    const float c02 = a01 * a12 - b11 * a02;
+    const float det = (b00 * c00 - a01 * c01 + a02 * c02) / (p * p * p);
+    const float hd = fminf(fmaxf(0.5f * det, -1.0f), 1.0f);
+    const float ang = acosf(hd) * (1.0f / 3.0f);
+    // separated root: the largest when hd >= 0, else the smallest
+    const float beta = hd >= 0.0f ? 2.0f * cosf(ang) : 2.0f * cosf(ang + 2.0943951023931955f);
+    const float ev = q + p * beta;
+    float r0[3] = {a00 - ev, a01, a02}, r1[3] = {a01, a11 - ev, a12}, r2[3] = {a02, a12, a22 - ev};
+    float x01[3] = {r0[1] * r1[2] - r0[2] * r1[1], r0[2] * r1[0] - r0[0] * r1[2], r0[0] * r1[1] - r0[1] * r1[0]};
+    float x02[3] = {r0[1] * r2[2] - r0[2] * r2[1], r0[2] * r2[0] - r0[0] * r2[2], r0[0] * r2[1] - r0[1] * r2[0]};
+    float x12[3] = {r1[1] * r2[2] - r1[2] * r2[1], r1[2] * r2[0] - r1[0] * r2[2], r1[0] * r2[1] - r1[1] * r2[0]};
+    float d0 = x01[0] * x01[0] + x01[1] * x01[1] + x01[2] * x01[2];
+    float d1 = x02[0] * x02[0] + x02[1] * x02[1] + x02[2] * x02[2];
+    float d2 = x12[0] * x12[0] + x12[1] * x12[1] + x12[2] * x12[2];
+    float w[3];
+    if (d0 >= d1 && d0 >= d2) {
+        float s = rsqrtf(d0);
+        w[0] = x01[0] * s; w[1] = x01[1] * s; w[2] = x01[2] * s;
+    } else if (d1 >= d2) {
+        float s = rsqrtf(d1);
+        w[0] = x02[0] * s; w[1] = x02[1] * s; w[2] = x02[2] * s;
+    } else {
+        float s = rsqrtf(d2);
+        w[0] = x12[0] * s; w[1] = x12[1] * s; w[2] = x12[2] * s;
+    }
+    float u[3];
+    if (fabsf(w[0]) > fabsf(w[1])) {
+        float s = rsqrtf(w[0] * w[0] + w[2] * w[2]);
+        u[0] = -w[2] * s; u[1] = 0.0f; u[2] = w[0] * s;
+    } else {
+        float s = rsqrtf(w[1] * w[1] + w[2] * w[2]);
+        u[0] = 0.0f; u[1] = w[2] * s; u[2] = -w[1] * s;
+    }
+    float v[3] = {w[1] * u[2] - w[2] * u[1], w[2] * u[0] - w[0] * u[2], w[0] * u[1] - w[1] * u[0]};
+    const float am[9] = {a00, a01, a02, a01, a11, a12, a02, a12, a22};
+    float au[3], av[3], aw[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        au[i] = am[3 * i] * u[0] + am[3 * i + 1] * u[1] + am[3 * i + 2] * u[2];
+        av[i] = am[3 * i] * v[0] + am[3 * i + 1] * v[1] + am[3 * i + 2] * v[2];
+        aw[i] = am[3 * i] * w[0] + am[3 * i + 1] * w[1] + am[3 * i + 2] * w[2];
+    }
+    const float m00 = u[0] * au[0] + u[1] * au[1] + u[2] * au[2];
+    const float m01 = u[0] * av[0] + u[1] * av[1] + u[2] * av[2];
+    const float m11 = v[0] * av[0] + v[1] * av[1] + v[2] * av[2];
+    const float th = 0.5f * atan2f(2.0f * m01, m00 - m11);
+    float sn, cs;
+    sincosf(th, &sn, &cs);
+    float p1[3], p2[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        p1[k] = u[k] * cs + v[k] * sn;
+        p2[k] = v[k] * cs - u[k] * sn;
+    }
+    // Rayleigh quotients; cs^2 m00 + 2 sn cs m01 + sn^2 m11 etc.
+    const float l0 = w[0] * aw[0] + w[1] * aw[1] + w[2] * aw[2];
+    const float l1 = cs * cs * m00 + 2.0f * sn * cs * m01 + sn * sn * m11;
+    const float l2 = sn * sn * m00 - 2.0f * sn * cs * m01 + cs * cs * m11;
+    vals[0] = l0 * mx; vals[1] = l1 * mx; vals[2] = l2 * mx;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        vecs[k] = w[k];
+        vecs[3 + k] = p1[k];
+        vecs[6 + k] = p2[k];
+    }
+}
+
+// Hencky elastic Kirchhoff stress (src/material.rs:708-771 with no
+// plasticity) from the displacement gradient Hd = F - I: principal
+// stretches of b = F F^T from the small quantity b - I = Hd + Hd^T + Hd Hd^T,
+// log strains by log1p, tau = K ln J I + 2G dev(eps).
+__device__ __forceinline__ void hencky_elastic(const MatParams& mp, const float* hd, float* tau) {
+    float bm[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            bm[3 * i + j] = hd[3 * i + j] + hd[3 * j + i] +
+                            (hd[3 * i] * hd[3 * j] + hd[3 * i + 1] * hd[3 * j + 1] + hd[3 * i + 2] * hd[3 * j + 2]);
+    float lam[3], vec[9];
+    sym_eigen3f(bm, lam, vec);
+    float eps[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) eps[a] = 0.5f * log1pf(fmaxf(lam[a], -0.9999999f));
+    const float vol = eps[0] + eps[1] + eps[2];
+    const float pressure = mp.bulk * vol;
+    const float g2 = 2.0f * mp.shear;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) tau[k] = (k == 0 || k == 4 || k == 8) ? pressure : 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float cf = g2 * (eps[a] - vol * (1.0f / 3.0f));
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) tau[3 * i + j] += cf * vec[3 * a + i] * vec[3 * a + j];
+    }
+}
+
+// Nodal forces of one element from tau (src/kernel.rs:141-154,
+// src/kernel_simd.rs:445-484): f_i = tau F^-T grad_i V + hourglass.
+// dl[8][3]: displacements relative to node 0; g, gam, hs read by the
+// caller's accessors.
+template <class GradF, class GamF, class HF>
+__device__ __forceinline__ void element_nodal_forces(const float* tau, const float* F, float J, float vol,
+                                                     const float (*dl)[3], GradF grad, GamF gam, HF hsym,
+                                                     float (*fo)[3]) {
+    float finv[9];
+    inv3f(F, J, finv);
+    // M = tau F^-T V
+    float M[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            M[3 * i + j] = vol * (tau[3 * i] * finv[3 * j] + tau[3 * i + 1] * finv[3 * j + 1] + tau[3 * i + 2] * finv[3 * j + 2]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const float g0 = grad(3 * i), g1 = grad(3 * i + 1), g2 = grad(3 * i + 2);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) fo[i][a] = M[3 * a] * g0 + M[3 * a + 1] * g1 + M[3 * a + 2] * g2;
+    }
+    // hourglass: q = U Gamma, r = H q, f += r Gamma^T (node 0 has dl = 0)
+    float q[12];
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            float s = 0.0f;
+#pragma unroll
+            for (int i = 1; i < 8; ++i) s += dl[i][d] * gam(4 * i + a);
+            q[3 * a + d] = s;
+        }
+    float rr[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) rr[k] = 0.0f;
+    int kk = 0;
+#pragma unroll
+    for (int r = 0; r < 12; ++r) {
+#pragma unroll
+        for (int c = r; c < 12; ++c) {
+            const float h = hsym(kk++);
+            rr[r] += h * q[c];
+            if (c != r) rr[c] += h * q[r];
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            float s = fo[i][d];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) s += rr[3 * a + d] * gam(4 * i + a);
+            fo[i][d] = s;
+        }
+}
+
+// ------------------------------------------------------ contact (f64)
+
+// The reference's acceptance test of node p against one face
+// (src/contact.rs:116-128): AABB reject, signed distance window, both
+// diagonal splits with the 10 % barycentric tolerance.  Returns true and
+// the signed distance when accepted.  Compiled without FMA contraction so
+// the decision is the IEEE f64 result of the reference's expression.
+__device__ __forceinline__ bool point_in_tri(const double* p, const double* a, const double* b, const double* c) {
+#pragma clang fp contract(off)
+    double v0[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    double v1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    double v2[3] = {p[0] - a[0], p[1] - a[1], p[2] - a[2]};
+    double d00 = v0[0] * v0[0] + v0[1] * v0[1] + v0[2] * v0[2];
+    double d01 = v0[0] * v1[0] + v0[1] * v1[1] + v0[2] * v1[2];
+    double d02 = v0[0] * v2[0] + v0[1] * v2[1] + v0[2] * v2[2];
+    double d11 = v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2];
+    double d12 = v1[0] * v2[0] + v1[1] * v2[1] + v1[2] * v2[2];
+    double inv = 1.0 / (d00 * d11 - d01 * d01);
+    double u = (d11 * d02 - d01 * d12) * inv;
+    double v = (d00 * d12 - d01 * d02) * inv;
+    return u >= -C_TRI_TOL && v >= -C_TRI_TOL && u + v <= 1.0 + C_TRI_TOL;
+}
+
+__device__ __forceinline__ bool face_accepts(const double* __restrict__ fg, int F, int f, const double* p,
+                                             double max_d, double* signed_out) {
+#pragma clang fp contract(off)
+    auto at = [&](int k) { return fg[(size_t)k * F + f]; };
+    if (p[0] < at(FG_LO) || p[0] > at(FG_HI) || p[1] < at(FG_LO + 1) || p[1] > at(FG_HI + 1) ||
+        p[2] < at(FG_LO + 2) || p[2] > at(FG_HI + 2))
+        return false;
+    const double n[3] = {at(FG_N), at(FG_N + 1), at(FG_N + 2)};
+    const double cp[3] = {at(FG_C) - p[0], at(FG_C + 1) - p[1], at(FG_C + 2) - p[2]};
+    const double sd = -(n[0] * cp[0] + n[1] * cp[1] + n[2] * cp[2]);
+    if (sd > 0.0 || sd < -max_d) return false;
+    const double pr[3] = {p[0] - n[0] * sd, p[1] - n[1] * sd, p[2] - n[2] * sd};
+    double x[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) x[k][d] = at(3 * k + d);
+    bool inside = point_in_tri(pr, x[0], x[1], x[2]) || point_in_tri(pr, x[0], x[2], x[3]) ||
+                  point_in_tri(pr, x[1], x[2], x[3]) || point_in_tri(pr, x[1], x[3], x[0]);
+    *signed_out = sd;
+    return inside;
+}
+
+// Face geometry for the current configuration (src/contact.rs:95-110) plus
+// the candidate box used by the bucket search: a superset of every point
+// the acceptance test can accept.  For each of the four triangles the
+// region of projected points it accepts is the barycentric-enlarged
+// triangle mapped onto the face plane along the triangle normal (an affine
+// map, so the corners suffice); the node itself lies up to max_distance
+// behind that.  Ill-conditioned faces fall back to the inflated AABB, which
+// the acceptance test applies first anyway.
+__device__ __forceinline__ void face_geometry(const double (*x)[3], double max_d, double* out /* [FG] */) {
+#pragma clang fp contract(off)
+    double e1[3], e2[3], n[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        e1[d] = x[1][d] - x[0][d];
+        e2[d] = x[2][d] - x[0][d];
+    }
+    n[0] = e1[1] * e2[2] - e1[2] * e2[1];
+    n[1] = e1[2] * e2[0] - e1[0] * e2[2];
+    n[2] = e1[0] * e2[1] - e1[1] * e2[0];
+    double nn = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) out[3 * k + d] = x[k][d];
+    double size = 0.0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        n[d] = n[d] / nn;
+        out[FG_N + d] = n[d];
+        out[FG_C + d] = (x[0][d] + x[1][d] + x[2][d] + x[3][d]) / 4.0;
+        double lo = fmin(fmin(x[0][d], x[1][d]), fmin(x[2][d], x[3][d]));
+        double hi = fmax(fmax(x[0][d], x[1][d]), fmax(x[2][d], x[3][d]));
+        out[FG_LO + d] = lo - max_d;
+        out[FG_HI + d] = hi + max_d;
+        size = fmax(size, hi - lo);
+    }
+    out[FG_SZ] = size;
+    // tight candidate region
+    const int tri[4][3] = {{0, 1, 2}, {0, 2, 3}, {1, 2, 3}, {1, 3, 0}};
+    const double cu[3] = {-C_TRI_TOL, 1.0 + 2.0 * C_TRI_TOL, -C_TRI_TOL};
+    const double cv[3] = {-C_TRI_TOL, -C_TRI_TOL, 1.0 + 2.0 * C_TRI_TOL};
+    double tlo[3] = {INFINITY, INFINITY, INFINITY}, thi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    bool valid = isfinite(nn) && nn > 0.0;
+    for (int t = 0; t < 4 && valid; ++t) {
+        const double* a = x[tri[t][0]];
+        const double* b = x[tri[t][1]];
+        const double* c = x[tri[t][2]];
+        double v0[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+        double v1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+        double nt[3] = {v1[1] * v0[2] - v1[2] * v0[1], v1[2] * v0[0] - v1[0] * v0[2], v1[0] * v0[1] - v1[1] * v0[0]};
+        double ntn = sqrt(nt[0] * nt[0] + nt[1] * nt[1] + nt[2] * nt[2]);
+        double l0 = sqrt(v0[0] * v0[0] + v0[1] * v0[1] + v0[2] * v0[2]);
+        double l1 = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+        if (!(ntn > 1e-6 * l0 * l1)) {
+            valid = false;
+            break;
+        }
+        double den = (n[0] * nt[0] + n[1] * nt[1] + n[2] * nt[2]) / ntn;
+        if (!(fabs(den) > 0.05)) {
+            valid = false;
+            break;
+        }
+        for (int k = 0; k < 3; ++k) {
+            double ep[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) ep[d] = a[d] + cu[k] * v0[d] + cv[k] * v1[d];
+            double tt = (n[0] * (out[FG_C] - ep[0]) + n[1] * (out[FG_C + 1] - ep[1]) + n[2] * (out[FG_C + 2] - ep[2])) /
+                        (den * ntn);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                double q = ep[d] + tt * nt[d];
+                double q2 = q - max_d * n[d];
+                tlo[d] = fmin(tlo[d], fmin(q, q2));
+                thi[d] = fmax(thi[d], fmax(q, q2));
+            }
+        }
+    }
+    const double margin = 1e-6 * size + 1e-12;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double lo = out[FG_LO + d], hi = out[FG_HI + d];
+        if (valid) {
+            lo = fmax(lo, tlo[d] - margin);
+            hi = fmin(hi, thi[d] + margin);
+        }
+        out[FG_CLO + d] = lo;
+        out[FG_CHI + d] = hi;
+    }
+}
+
+// V / A_max of a hexahedron from 8 positions (src/element.rs:176-186), f64.
+__device__ __forceinline__ void char_length_d(const double (*p)[3], double* length, double* volume) {
+#pragma clang fp contract(off)
+    const int tets[6][4] = {{0, 1, 2, 6}, {0, 2, 3, 6}, {0, 3, 7, 6}, {0, 7, 4, 6}, {0, 4, 5, 6}, {0, 5, 1, 6}};
+    const int faces[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}};
+    double vol = 0.0;
+#pragma unroll
+    for (int t = 0; t < 6; ++t) {
+        double u[3], v[3], w[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            u[d] = p[tets[t][1]][d] - p[tets[t][0]][d];
+            v[d] = p[tets[t][2]][d] - p[tets[t][0]][d];
+            w[d] = p[tets[t][3]][d] - p[tets[t][0]][d];
+        }
+        double cx = u[1] * v[2] - u[2] * v[1], cy = u[2] * v[0] - u[0] * v[2], cz = u[0] * v[1] - u[1] * v[0];
+        vol += (cx * w[0] + cy * w[1] + cz * w[2]) / 6.0;
+    }
+    double amax = 0.0;
+#pragma unroll
+    for (int f = 0; f < 6; ++f) {
+        double a[3], b[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            a[d] = p[faces[f][2]][d] - p[faces[f][0]][d];
+            b[d] = p[faces[f][3]][d] - p[faces[f][1]][d];
+        }
+        double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
+        amax = fmax(amax, 0.5 * sqrt(cx * cx + cy * cy + cz * cz));
+    }
+    *volume = vol;
+    *length = (vol <= 0.0 || amax <= 0.0) ? INFINITY : vol / amax;
+}
+
+}  // namespace crush

@@ -1,0 +1,89 @@
+// Host-side preparation of a model for the device: validation, element
+// geometry classes, node/element incidence (CSR), contact tables and the
+// initial time step.  Shared by the one-solve path, the batched path and
+// the server.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+#include "../../include/crushrs.h"
+
+namespace crush {
+
+constexpr int HSYM = 78;        // packed upper triangle of the 12x12 H
+constexpr int SEARCH_BLOCK = 256;
+constexpr int FACE_HITS_CAP = 16;  // recorded hits per face before the scan fallback
+
+// Per-material parameters as the force kernels read them (f32).
+struct MatParams {
+    int32_t kind;        // CRUSH_MAT_ELASTIC or CRUSH_MAT_HONEYCOMB
+    float youngs;
+    float bulk;          // elastic: K = E / (3 (1 - 2 nu))
+    float shear;         // elastic: G = E / (2 (1 + nu))
+    float trans;         // honeycomb: transverse cap factor
+    float knot_c[8];     // honeycomb knot table (unused knots at +inf)
+    float knot_s[8];
+    float knot_h[8];
+};
+
+struct ContactDesc {
+    int32_t node_begin, n_nodes;   // into the flattened contact-node arrays
+    int32_t face_begin, n_faces;   // into the flattened contact-face arrays
+    double stiffness, max_distance, depth_cap;
+    int32_t pair, reserved;
+};
+
+struct Prepared {
+    int n_nodes = 0, n_elems = 0, n_hc = 0, n_classes = 0, n_parts = 0;
+    // Element slots: honeycomb elements first, then elastic, each in
+    // original order.
+    std::vector<int32_t> slot_elem, elem_slot;
+    std::vector<int32_t> conn;        // [8][E] by slot
+    std::vector<int32_t> ecls;        // [E] geometry class per slot
+    std::vector<float> cls_grad;      // [24][C]
+    std::vector<float> cls_vol;       // [C]
+    std::vector<float> cls_gamma;     // [32][C]
+    std::vector<float> cls_H;         // [78][C]
+    std::vector<int32_t> cls_mat;     // [C]
+    std::vector<double> cls_dtf;      // [C] (1 - 0.36 nu) / c_d
+    std::vector<double> elem_mass;    // [E] by slot: V rho
+    std::vector<MatParams> mats;      // [parts]
+    std::vector<double> X, v0, mass;  // [3N], [3N], [N]
+    std::vector<uint8_t> fixed;       // [N]
+    // node -> (slot << 3 | corner), incidences in original element order
+    std::vector<int32_t> n2e_ptr, n2e_idx;
+    // contacts
+    std::vector<ContactDesc> cdesc;
+    std::vector<int32_t> cn_node;     // [K] kept contacting nodes
+    std::vector<int32_t> cn_contact;  // [K]
+    std::vector<double> cn_scale;     // [K]
+    std::vector<int32_t> cf_nodes;    // [4][F] node ids of each contact face (SoA)
+    std::vector<int32_t> cf_contact;  // [F]
+    std::vector<int32_t> tn_node, tn_ptr, tn_ent;  // contact apply lists
+    std::vector<int32_t> sblk_contact, sblk_start;  // search blocks
+    int n_cnodes = 0, n_cfaces = 0;
+    // time step
+    double contact_dt = 0, stable = 0, dt_floor = 0, dt0 = 0;
+    bool adaptive = true;
+    crush_settings settings{};
+    // groups (outputs)
+    int n_groups = 0;
+    std::vector<int32_t> g_nodes_ptr, g_nodes, g_front_ptr, g_front, g_accel, g_axis;
+    // total mass and lumped masses of each group (for outputs)
+    std::vector<double> g_mass, g_v0;  // g_v0: [3*G] initial mean velocity
+};
+
+// Validate and prepare; throws crush::Error.
+void prepare(const crush_model& m, Prepared& p);
+
+// Element geometry of one hexahedron (product implementation): centre
+// gradient [24], volume, lumped masses [8] (unit density), hourglass base
+// [32] ([4*i+a]) and the packed 12x12 stiffness [78] for E = 1.
+void hex_geometry(const double* x, double poisson, double hourglass, double* grad, double* vol,
+                  double* lumped, double* gamma, double* hsym);
+
+// Characteristic length V / A_max and volume of a hexahedron (f64).
+void hex_char_length(const double* p, double* length, double* volume);
+
+}  // namespace crush

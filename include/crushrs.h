@@ -1,0 +1,196 @@
+/*
+ * crushrs.h — C ABI of the MI355X crash-solver core.
+ *
+ * The reference crate (osullivryan/crushrs) exposes one solver entry point,
+ * `pub fn run(model: &Model) -> Results` (src/solver.rs:190, re-exported as
+ * `crushrs::run` in src/lib.rs:26).  This header is the drop-in boundary for
+ * that call: plain C structs of caller-owned arrays in, caller-owned arrays
+ * out, integer status codes instead of panics.  Everything behind it (mesh
+ * classes, HIP kernels, device memory) is private to libcrushrs_amd.so.
+ *
+ * Conventions
+ *   - Node, element and face indices are 0-based int32.
+ *   - Vectors are interleaved xyz: v[3*node + axis].
+ *   - Every pointer in an input struct is borrowed for the duration of the
+ *     call only; the library copies what it needs to the device.
+ *   - Functions return CRUSH_OK (0) or a CRUSH_E_* code; the message of the
+ *     last failure on the calling thread is crush_last_error().
+ */
+#ifndef CRUSHRS_H
+#define CRUSHRS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CRUSH_ABI_VERSION 1
+
+/* Status codes.  The reference panics in all of these cases
+ * (src/solver.rs:302 non-finite displacement, src/model.rs:186-242 unknown
+ * sets, src/material.rs:188,432 invalid curves). */
+#define CRUSH_OK            0
+#define CRUSH_E_INPUT       1  /* malformed model, settings or arguments       */
+#define CRUSH_E_NONFINITE   2  /* displacement became non-finite               */
+#define CRUSH_E_OOM         3  /* device or host allocation failed             */
+#define CRUSH_E_DEVICE      4  /* HIP runtime error / no device                */
+#define CRUSH_E_UNSUPPORTED 5  /* material model not on the GPU path           */
+#define CRUSH_E_BUSY        6  /* server full (admission refused)              */
+#define CRUSH_E_NOTFOUND    7  /* unknown vehicle / file / ticket              */
+
+/* Material models: src/material.rs:20-36 (PlasticModel) plus plain elastic
+ * (Material with plasticity = None, src/material.rs:165).  The GPU path runs
+ * ELASTIC (finite-strain Hencky, src/material.rs:708-771, elastic branch) and
+ * HONEYCOMB (corotational rate form, src/material.rs:472-535). */
+#define CRUSH_MAT_ELASTIC   0
+#define CRUSH_MAT_HONEYCOMB 1
+#define CRUSH_MAT_J2        2
+#define CRUSH_MAT_FOAM      3
+
+#define CRUSH_MAX_KNOTS 8  /* src/material.rs:39 MAX_KNOTS */
+
+/* One part's material: src/material.rs:41-71 (Plasticity) and :154-162. */
+typedef struct crush_material {
+    double youngs_modulus;
+    double poisson_ratio;
+    double density;
+    int32_t model;            /* CRUSH_MAT_*                                   */
+    int32_t n_curve;          /* 0 (bilinear) or 2..8 tabulated knots          */
+    double yield_stress;      /* bilinear: initial yield stress (Pa)           */
+    double hardening;         /* bilinear: hardening modulus (Pa)              */
+    double curve[CRUSH_MAX_KNOTS][2]; /* [compaction, yield stress] knots      */
+    int32_t has_densification;
+    int32_t reserved0;
+    double densification[2];  /* [c_lock, k_lock] lock-up                      */
+    double transverse_factor; /* cap factor on local y, z normal stresses      */
+} crush_material;
+
+/* One-way node-to-face penalty contact: src/model.rs:12-28 (Contact).
+ * add_contact_pair (src/model.rs:250-265) is two of these at half stiffness. */
+typedef struct crush_contact {
+    int32_t n_nodes;
+    int32_t n_faces;
+    const int32_t* nodes;      /* contacting nodes                             */
+    const double* node_scale;  /* per-node stiffness factor, NULL = all 1      */
+    const int32_t* faces;      /* indices into crush_model.faces               */
+    double stiffness;          /* N/m per node                                 */
+    double max_distance;       /* m                                            */
+    int32_t pair;              /* 1: vehicle-to-vehicle (used by separation)   */
+    int32_t reserved0;
+} crush_contact;
+
+/* src/model.rs:75-150 (Settings).  Fields the GPU core does not use
+ * (frame_steps, history_steps, simd, threads) belong to the output layer. */
+typedef struct crush_settings {
+    double end_time;             /* s (reference default 0.15)                  */
+    double dt_scale;             /* safety factor on the stable step (0.85)     */
+    double time_step;            /* fixed step if > 0, else adaptive            */
+    double hourglass;            /* stabilisation stiffness / E (0.1)           */
+    double velocity_damping;     /* per-step velocity factor (1.0)              */
+    double mass_scaling;         /* dt floor as fraction of initial step (0.25) */
+    int32_t adaptive_check_steps;/* re-evaluate dt every N steps (5)            */
+    /* Retire at separation (no counterpart in the reference, which always
+     * runs to end_time): 0 = off.  When > 0 the solve stops after a check
+     * step (every adaptive_check_steps) at which (a) pair contacts have had
+     * no penetrating node for this many consecutive steps, (b) they had
+     * one before, and (c) groups 0 and 1 move apart:
+     * (v1 - v0) . (x1 - x0) > 0 with mass-weighted group means. */
+    int32_t separation_steps;
+} crush_settings;
+
+/* A vehicle, for the delta-v / crush outputs of src/bin/crushrs.rs:237-242
+ * (part_delta_v) and :265-271 (crush_rear). */
+typedef struct crush_group {
+    int32_t n_nodes;
+    int32_t n_front;
+    const int32_t* nodes;      /* node set <name> (all parts of the vehicle)   */
+    const int32_t* front;      /* nodes of face set <name>_front               */
+    int32_t accel_node;        /* rear-seat accelerometer node, -1 = none      */
+    int32_t axis;              /* crush axis, 0 = x                            */
+} crush_group;
+
+/* src/model.rs:152-165 (Model) + src/mesh.rs:37-52 (Mesh), flattened. */
+typedef struct crush_model {
+    int32_t n_nodes;
+    int32_t n_hexes;
+    int32_t n_parts;
+    int32_t n_faces;
+    int32_t n_fixed;
+    int32_t n_contacts;
+    int32_t n_groups;
+    int32_t reserved0;
+    const double* nodes;             /* [3*n_nodes] reference positions       */
+    const int32_t* hexes;            /* [8*n_hexes] C3D8 ordering              */
+    const int32_t* hex_part;         /* [n_hexes]                              */
+    const crush_material* materials; /* [n_parts]                              */
+    const double* initial_velocity;  /* [3*n_nodes]                            */
+    const int32_t* fixed_nodes;      /* [n_fixed]                              */
+    const int32_t* faces;            /* [4*n_faces] quads, CCW about normal    */
+    const crush_contact* contacts;   /* [n_contacts]                           */
+    const crush_group* groups;       /* [n_groups]                             */
+    crush_settings settings;
+} crush_model;
+
+typedef struct crush_group_result {
+    double mass;           /* lumped mass of the node set                     */
+    double delta_v[3];     /* final minus initial mass-weighted mean velocity */
+    double crush_end;      /* |u_axis(accel) - mean u_axis(front)| at the end */
+    double crush_max;      /* max of the same over all steps                  */
+} crush_group_result;
+
+/* src/solver.rs:63-80 (Results).  Output arrays are caller-owned; pass NULL
+ * for any the caller does not want. */
+typedef struct crush_results {
+    double time;                 /* simulated time reached (s)                 */
+    int64_t steps;
+    int64_t eroded_elements;     /* elements eroded (inverted) during the run  */
+    double wall_ms;              /* host wall time of the solve                */
+    double device_ms;            /* device time of the step loop               */
+    double added_mass_max;       /* largest total mass added by mass scaling   */
+    int32_t separated;           /* 1 if retired at separation                 */
+    int32_t n_groups;            /* entries written to groups                  */
+    double* displacement;        /* [3*n_nodes] or NULL                        */
+    double* velocity;            /* [3*n_nodes] or NULL                        */
+    double* masses;              /* [n_nodes] lumped masses, or NULL           */
+    crush_group_result* groups;  /* [n_groups] or NULL                         */
+} crush_results;
+
+/* ---- core entry points ------------------------------------------------- */
+
+/* Library/ABI version (CRUSH_ABI_VERSION). */
+int32_t crush_abi_version(void);
+
+/* Message of the last failure on this thread ("" if none). */
+const char* crush_last_error(void);
+
+/* Number of visible HIP devices (0 when none); never fails. */
+int32_t crush_device_count(void);
+
+/* The explicit solve on the GPU, one model at a time.
+ * Replaces crushrs::run (src/solver.rs:190-381, src/lib.rs:26). */
+int32_t crush_run(const crush_model* model, crush_results* out);
+
+/* Parity taps: run the first n_steps steps of the solve on the GPU and
+ * record, per step s (0-based, after the step), f_int[s*3N..] (internal
+ * nodal forces at the new configuration, src/solver.rs:327-328),
+ * f_ext[s*3N..] (contact forces used by that step, src/solver.rs:307-310),
+ * hit[s*n_contact_nodes..] (face index hit by each contacting node of each
+ * contact, -1 = none, contacts concatenated in order, nodes in contact
+ * order after the static broad phase of src/contact.rs:48-70, i.e. only
+ * nodes kept by it), eroded[s*n_hexes..] (1 = eroded) and dt[s].
+ * Any output pointer may be NULL.  Returns the number of steps taken in
+ * *steps_done (fewer than n_steps if end_time was reached). */
+int32_t crush_debug_run_steps(const crush_model* model, int32_t n_steps,
+                              double* f_int, double* f_ext, int32_t* hit,
+                              uint8_t* eroded, double* dt, int32_t* steps_done);
+
+/* Number of contacting nodes the static broad phase keeps, summed over the
+ * contacts (the row length of crush_debug_run_steps' hit). */
+int32_t crush_contact_node_count(const crush_model* model, int32_t* count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CRUSHRS_H */
