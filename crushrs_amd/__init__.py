@@ -260,7 +260,7 @@ def debug_run_steps(model: Model, n_steps: int):
     dt = np.zeros(n_steps)
     done = C.c_int32()
     check(lib.crush_debug_run_steps(model.ptr, n_steps, fi.ctypes.data_as(_abi.f64p), fx.ctypes.data_as(_abi.f64p),
-                                    hit.ctypes.data_as(_abi.i32p), er.ctypes.data_as(_abi.u8p),
+                                    None, hit.ctypes.data_as(_abi.i32p), er.ctypes.data_as(_abi.u8p),
                                     dt.ctypes.data_as(_abi.f64p), C.byref(done)))
     s = done.value
     return fi[:s], fx[:s], hit[:s, :k], er[:s], dt[:s]

@@ -37,11 +37,11 @@ int32_t crush_run(const crush_model* model, crush_results* out) {
     });
 }
 
-int32_t crush_debug_run_steps(const crush_model* model, int32_t n_steps, double* f_int, double* f_ext, int32_t* hit,
-                              uint8_t* eroded, double* dt, int32_t* steps_done) {
+int32_t crush_debug_run_steps(const crush_model* model, int32_t n_steps, double* f_int, double* f_ext, double* u,
+                              int32_t* hit, uint8_t* eroded, double* dt, int32_t* steps_done) {
     return guarded([&] {
         if (!model || n_steps < 0) fail(CRUSH_E_INPUT, "bad arguments");
-        run_plain_taps(*model, n_steps, f_int, f_ext, hit, eroded, dt, steps_done);
+        run_plain_taps(*model, n_steps, f_int, f_ext, u, hit, eroded, dt, steps_done);
     });
 }
 

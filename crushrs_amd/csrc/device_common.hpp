@@ -7,10 +7,23 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <limits>
 
 #include "prep.hpp"
 
+// Element math is also compiled for the host so that tests can exercise it
+// on the CPU; device code uses the hardware reciprocal square root.
+#define CRUSH_HD __host__ __device__ __forceinline__
+
 namespace crush {
+
+CRUSH_HD float rsqrt_f(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return rsqrtf(x);
+#else
+    return 1.0f / sqrtf(x);
+#endif
+}
 
 // Per-solve control block in device memory.
 struct StepCtl {
@@ -39,13 +52,13 @@ constexpr double C_TRI_TOL = 0.1;
 
 // ------------------------------------------------------------ f32 3x3
 
-__device__ __forceinline__ float det3f(const float* m) {
+CRUSH_HD float det3f(const float* m) {
     return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
            m[2] * (m[3] * m[7] - m[4] * m[6]);
 }
 
 // Cofactor inverse with a given determinant.
-__device__ __forceinline__ void inv3f(const float* m, float det, float* r) {
+CRUSH_HD void inv3f(const float* m, float det, float* r) {
     float id = 1.0f / det;
     r[0] = (m[4] * m[8] - m[5] * m[7]) * id;
     r[1] = (m[2] * m[7] - m[1] * m[8]) * id;
@@ -58,7 +71,7 @@ __device__ __forceinline__ void inv3f(const float* m, float det, float* r) {
     r[8] = (m[0] * m[4] - m[1] * m[3]) * id;
 }
 
-__device__ __forceinline__ void mul3f(const float* a, const float* b, float* r) {
+CRUSH_HD void mul3f(const float* a, const float* b, float* r) {
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -68,16 +81,26 @@ __device__ __forceinline__ void mul3f(const float* a, const float* b, float* r) 
 // ------------------------------------------------- element force (f32)
 
 // Honeycomb rate-form stress update (src/material.rs:472-535, the f32
-// lane form of src/kernel_simd.rs:336-443): F the deformation gradient,
-// st[25] = F_prev^-1 (9), R (9), sigma_hat (6), compaction; updates st and
-// returns the Kirchhoff stress tau (row-major).
-__device__ __forceinline__ void honeycomb_update(const MatParams& mp, const float* F, float* st, float* tau) {
-    float finc[9], mid[9], midinv[9], g[9], dinc[9];
-    mul3f(F, st, finc);
+// lane form of src/kernel_simd.rs:336-443).  hd = F - I is the current
+// displacement gradient; st[25] = H_prev (9, the displacement gradient at
+// the last committed step), R (9), sigma_hat (6), compaction.  The
+// reference's F_inc - I = F F_prev^-1 - I is formed here as
+// (H - H_prev) F_prev^-1, the same quantity without the cancellation
+// against I that limits an f32 F_prev^-1.  Updates st and returns the
+// Kirchhoff stress tau (row-major).
+CRUSH_HD void honeycomb_update(const MatParams& mp, const float* hd, const float* F, float* st,
+                                                 float* tau) {
+    float fprev[9], fpinv[9], dh[9], mid[9], midinv[9], g[9], dinc[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        fprev[k] = st[k] + ((k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f);
+        dh[k] = hd[k] - st[k];
+    }
+    inv3f(fprev, det3f(fprev), fpinv);
+    mul3f(dh, fpinv, dinc);
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         float id = (k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f;
-        dinc[k] = finc[k] - id;
         mid[k] = 0.5f * dinc[k] + id;
     }
     inv3f(mid, det3f(mid), midinv);
@@ -161,7 +184,8 @@ __device__ __forceinline__ void honeycomb_update(const MatParams& mp, const floa
         for (int j = 0; j < 3; ++j)
             tau[3 * i + j] = J * (rs[3 * i] * r[3 * j] + rs[3 * i + 1] * r[3 * j + 1] + rs[3 * i + 2] * r[3 * j + 2]);
     // commit
-    inv3f(F, J, st);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) st[k] = hd[k];
 #pragma unroll
     for (int k = 0; k < 9; ++k) st[9 + k] = r[k];
 #pragma unroll
@@ -169,93 +193,106 @@ __device__ __forceinline__ void honeycomb_update(const MatParams& mp, const floa
     st[24] = c + dc;
 }
 
+CRUSH_HD float rsqrt_t(float x) { return rsqrt_f(x); }
+CRUSH_HD double rsqrt_t(double x) { return 1.0 / sqrt(x); }
+
 // Symmetric 3x3 eigen-decomposition (vals ascending irrelevant here):
 // trigonometric root for the well-separated eigenvalue, its eigenvector
 // from the largest row cross product, one exact Jacobi rotation in the
 // orthogonal plane and Rayleigh-quotient eigenvalues (the method of
 // src/material.rs:570-659).  Works on any scale; vecs column-major
 // (vecs[3*c + i] = component i of eigenvector c).
-__device__ __forceinline__ void sym_eigen3f(const float* a, float* vals, float* vecs) {
-    float a00 = a[0], a01 = a[1], a02 = a[2], a11 = a[4], a12 = a[5], a22 = a[8];
-    float mx = fmaxf(fmaxf(fmaxf(fabsf(a00), fabsf(a01)), fmaxf(fabsf(a02), fabsf(a11))), fmaxf(fabsf(a12), fabsf(a22)));
-    if (mx == 0.0f) {
-        vals[0] = vals[1] = vals[2] = 0.0f;
+template <class T>
+CRUSH_HD void sym_eigen3(const T* a, T* vals, T* vecs) {
+    const T eps2 = std::numeric_limits<T>::epsilon() * std::numeric_limits<T>::epsilon();
+    T a00 = a[0], a01 = a[1], a02 = a[2], a11 = a[4], a12 = a[5], a22 = a[8];
+    T mx = fmax(fmax(fmax(fabs(a00), fabs(a01)), fmax(fabs(a02), fabs(a11))), fmax(fabs(a12), fabs(a22)));
+    if (!(mx >= std::numeric_limits<T>::min() * T(1e8))) {
+        vals[0] = vals[1] = vals[2] = T(0.0);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) vecs[k] = (k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f;
+        for (int k = 0; k < 9; ++k) vecs[k] = (k == 0 || k == 4 || k == 8) ? T(1.0) : T(0.0);
         return;
     }
-    const float inv = 1.0f / mx;
+    const T inv = T(1.0) / mx;
     a00 *= inv; a01 *= inv; a02 *= inv; a11 *= inv; a12 *= inv; a22 *= inv;
-    const float off = a01 * a01 + a02 * a02 + a12 * a12;
-    if (off == 0.0f) {
+    const T off = a01 * a01 + a02 * a02 + a12 * a12;
+    // off-diagonals below the resolution of the diagonal: treat as diagonal
+    if (off < eps2) {
         vals[0] = a00 * mx; vals[1] = a11 * mx; vals[2] = a22 * mx;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) vecs[k] = (k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f;
+        for (int k = 0; k < 9; ++k) vecs[k] = (k == 0 || k == 4 || k == 8) ? T(1.0) : T(0.0);
         return;
     }
-    const float q = (a00 + a11 + a22) * (1.0f / 3.0f);
-    const float b00 = a00 - q, b11 = a11 - q, b22 = a22 - q;
-    const float p = sqrtf((b00 * b00 + b11 * b11 + b22 * b22 + 2.0f * off) * (1.0f / 6.0f));
-    const float c00 = b11 * b22 - a12 * a12;
-    const float c01 = a01 * b22 - a12 * a02;
-    const float c02 = a01 * a12 - b11 * a02;
-    const float det = (b00 * c00 - a01 * c01 + a02 * c02) / (p * p * p);
-    const float hd = fminf(fmaxf(0.5f * det, -1.0f), 1.0f);
-    const float ang = acosf(hd) * (1.0f / 3.0f);
+    const T q = (a00 + a11 + a22) * (T(1.0) / T(3.0));
+    const T b00 = a00 - q, b11 = a11 - q, b22 = a22 - q;
+    const T p = sqrt((b00 * b00 + b11 * b11 + b22 * b22 + T(2.0) * off) * (T(1.0) / T(6.0)));
+    const T c00 = b11 * b22 - a12 * a12;
+    const T c01 = a01 * b22 - a12 * a02;
+    const T c02 = a01 * a12 - b11 * a02;
+    const T det = (b00 * c00 - a01 * c01 + a02 * c02) / (p * p * p);
+    const T hd = fmin(fmax(T(0.5) * det, -T(1.0)), T(1.0));
+    const T ang = acos(hd) * (T(1.0) / T(3.0));
     // separated root: the largest when hd >= 0, else the smallest
-    const float beta = hd >= 0.0f ? 2.0f * cosf(ang) : 2.0f * cosf(ang + 2.0943951023931955f);
-    const float ev = q + p * beta;
-    float r0[3] = {a00 - ev, a01, a02}, r1[3] = {a01, a11 - ev, a12}, r2[3] = {a02, a12, a22 - ev};
-    float x01[3] = {r0[1] * r1[2] - r0[2] * r1[1], r0[2] * r1[0] - r0[0] * r1[2], r0[0] * r1[1] - r0[1] * r1[0]};
-    float x02[3] = {r0[1] * r2[2] - r0[2] * r2[1], r0[2] * r2[0] - r0[0] * r2[2], r0[0] * r2[1] - r0[1] * r2[0]};
-    float x12[3] = {r1[1] * r2[2] - r1[2] * r2[1], r1[2] * r2[0] - r1[0] * r2[2], r1[0] * r2[1] - r1[1] * r2[0]};
-    float d0 = x01[0] * x01[0] + x01[1] * x01[1] + x01[2] * x01[2];
-    float d1 = x02[0] * x02[0] + x02[1] * x02[1] + x02[2] * x02[2];
-    float d2 = x12[0] * x12[0] + x12[1] * x12[1] + x12[2] * x12[2];
-    float w[3];
+    const T beta = hd >= T(0.0) ? T(2.0) * cos(ang) : T(2.0) * cos(ang + T(2.0943951023931955));
+    const T ev = q + p * beta;
+    T r0[3] = {a00 - ev, a01, a02}, r1[3] = {a01, a11 - ev, a12}, r2[3] = {a02, a12, a22 - ev};
+    T x01[3] = {r0[1] * r1[2] - r0[2] * r1[1], r0[2] * r1[0] - r0[0] * r1[2], r0[0] * r1[1] - r0[1] * r1[0]};
+    T x02[3] = {r0[1] * r2[2] - r0[2] * r2[1], r0[2] * r2[0] - r0[0] * r2[2], r0[0] * r2[1] - r0[1] * r2[0]};
+    T x12[3] = {r1[1] * r2[2] - r1[2] * r2[1], r1[2] * r2[0] - r1[0] * r2[2], r1[0] * r2[1] - r1[1] * r2[0]};
+    T d0 = x01[0] * x01[0] + x01[1] * x01[1] + x01[2] * x01[2];
+    T d1 = x02[0] * x02[0] + x02[1] * x02[1] + x02[2] * x02[2];
+    T d2 = x12[0] * x12[0] + x12[1] * x12[1] + x12[2] * x12[2];
+    T w[3];
+    if (!(fmax(d0, fmax(d1, d2)) > eps2 * eps2)) {
+        vals[0] = a00 * mx; vals[1] = a11 * mx; vals[2] = a22 * mx;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) vecs[k] = (k == 0 || k == 4 || k == 8) ? T(1.0) : T(0.0);
+        return;
+    }
     if (d0 >= d1 && d0 >= d2) {
-        float s = rsqrtf(d0);
+        T s = rsqrt_t(d0);
         w[0] = x01[0] * s; w[1] = x01[1] * s; w[2] = x01[2] * s;
     } else if (d1 >= d2) {
-        float s = rsqrtf(d1);
+        T s = rsqrt_t(d1);
         w[0] = x02[0] * s; w[1] = x02[1] * s; w[2] = x02[2] * s;
     } else {
-        float s = rsqrtf(d2);
+        T s = rsqrt_t(d2);
         w[0] = x12[0] * s; w[1] = x12[1] * s; w[2] = x12[2] * s;
     }
-    float u[3];
-    if (fabsf(w[0]) > fabsf(w[1])) {
-        float s = rsqrtf(w[0] * w[0] + w[2] * w[2]);
-        u[0] = -w[2] * s; u[1] = 0.0f; u[2] = w[0] * s;
+    T u[3];
+    if (fabs(w[0]) > fabs(w[1])) {
+        T s = rsqrt_t(w[0] * w[0] + w[2] * w[2]);
+        u[0] = -w[2] * s; u[1] = T(0.0); u[2] = w[0] * s;
     } else {
-        float s = rsqrtf(w[1] * w[1] + w[2] * w[2]);
-        u[0] = 0.0f; u[1] = w[2] * s; u[2] = -w[1] * s;
+        T s = rsqrt_t(w[1] * w[1] + w[2] * w[2]);
+        u[0] = T(0.0); u[1] = w[2] * s; u[2] = -w[1] * s;
     }
-    float v[3] = {w[1] * u[2] - w[2] * u[1], w[2] * u[0] - w[0] * u[2], w[0] * u[1] - w[1] * u[0]};
-    const float am[9] = {a00, a01, a02, a01, a11, a12, a02, a12, a22};
-    float au[3], av[3], aw[3];
+    T v[3] = {w[1] * u[2] - w[2] * u[1], w[2] * u[0] - w[0] * u[2], w[0] * u[1] - w[1] * u[0]};
+    const T am[9] = {a00, a01, a02, a01, a11, a12, a02, a12, a22};
+    T au[3], av[3], aw[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
         au[i] = am[3 * i] * u[0] + am[3 * i + 1] * u[1] + am[3 * i + 2] * u[2];
         av[i] = am[3 * i] * v[0] + am[3 * i + 1] * v[1] + am[3 * i + 2] * v[2];
         aw[i] = am[3 * i] * w[0] + am[3 * i + 1] * w[1] + am[3 * i + 2] * w[2];
     }
-    const float m00 = u[0] * au[0] + u[1] * au[1] + u[2] * au[2];
-    const float m01 = u[0] * av[0] + u[1] * av[1] + u[2] * av[2];
-    const float m11 = v[0] * av[0] + v[1] * av[1] + v[2] * av[2];
-    const float th = 0.5f * atan2f(2.0f * m01, m00 - m11);
-    float sn, cs;
-    sincosf(th, &sn, &cs);
-    float p1[3], p2[3];
+    const T m00 = u[0] * au[0] + u[1] * au[1] + u[2] * au[2];
+    const T m01 = u[0] * av[0] + u[1] * av[1] + u[2] * av[2];
+    const T m11 = v[0] * av[0] + v[1] * av[1] + v[2] * av[2];
+    const T th = T(0.5) * atan2(T(2.0) * m01, m00 - m11);
+    T sn, cs;
+    sn = sin(th);
+    cs = cos(th);
+    T p1[3], p2[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         p1[k] = u[k] * cs + v[k] * sn;
         p2[k] = v[k] * cs - u[k] * sn;
     }
     // Rayleigh quotients; cs^2 m00 + 2 sn cs m01 + sn^2 m11 etc.
-    const float l0 = w[0] * aw[0] + w[1] * aw[1] + w[2] * aw[2];
-    const float l1 = cs * cs * m00 + 2.0f * sn * cs * m01 + sn * sn * m11;
-    const float l2 = sn * sn * m00 - 2.0f * sn * cs * m01 + cs * cs * m11;
+    const T l0 = w[0] * aw[0] + w[1] * aw[1] + w[2] * aw[2];
+    const T l1 = cs * cs * m00 + T(2.0) * sn * cs * m01 + sn * sn * m11;
+    const T l2 = sn * sn * m00 - T(2.0) * sn * cs * m01 + cs * cs * m11;
     vals[0] = l0 * mx; vals[1] = l1 * mx; vals[2] = l2 * mx;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
@@ -265,11 +302,16 @@ __device__ __forceinline__ void sym_eigen3f(const float* a, float* vals, float* 
     }
 }
 
+CRUSH_HD void sym_eigen3f(const float* a, float* vals, float* vecs) { sym_eigen3<float>(a, vals, vecs); }
+
 // Hencky elastic Kirchhoff stress (src/material.rs:708-771 with no
-// plasticity) from the displacement gradient Hd = F - I: principal
-// stretches of b = F F^T from the small quantity b - I = Hd + Hd^T + Hd Hd^T,
-// log strains by log1p, tau = K ln J I + 2G dev(eps).
-__device__ __forceinline__ void hencky_elastic(const MatParams& mp, const float* hd, float* tau) {
+// plasticity): tau = K tr(eps) I + 2G dev(eps) with eps = ln(b)/2, b = F F^T.
+// The small quantity A = b - I = Hd + Hd^T + Hd Hd^T is formed from the
+// displacement gradient Hd = F - I.  For |A| < 0.05 (every elastic element of
+// a vehicle model in practice) eps = ln(I + A)/2 is the matrix series to A^6
+// (truncation below 2e-10 |A|); otherwise principal stretches from an f64
+// eigen-decomposition with eps_a = log1p(mu_a)/2.
+CRUSH_HD void hencky_elastic(const MatParams& mp, const float* hd, float* tau) {
     float bm[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -277,23 +319,59 @@ __device__ __forceinline__ void hencky_elastic(const MatParams& mp, const float*
         for (int j = 0; j < 3; ++j)
             bm[3 * i + j] = hd[3 * i + j] + hd[3 * j + i] +
                             (hd[3 * i] * hd[3 * j] + hd[3 * i + 1] * hd[3 * j + 1] + hd[3 * i + 2] * hd[3 * j + 2]);
-    float lam[3], vec[9];
-    sym_eigen3f(bm, lam, vec);
-    float eps[3];
+    float mx = 0.0f;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) eps[a] = 0.5f * log1pf(fmaxf(lam[a], -0.9999999f));
-    const float vol = eps[0] + eps[1] + eps[2];
-    const float pressure = mp.bulk * vol;
-    const float g2 = 2.0f * mp.shear;
+    for (int k = 0; k < 9; ++k) mx = fmaxf(mx, fabsf(bm[k]));
+    float eps[9];
+    if (mx < 0.05f) {
+        // ln(I + A) = A (I - A (I/2 - A (I/3 - A (I/4 - A (I/5 - A/6)))))
+        const float c[6] = {1.0f, 0.5f, 1.0f / 3.0f, 0.25f, 0.2f, 1.0f / 6.0f};
+        float acc[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) tau[k] = (k == 0 || k == 4 || k == 8) ? pressure : 0.0f;
+        for (int k = 0; k < 9; ++k) acc[k] = ((k == 0 || k == 4 || k == 8) ? c[4] : 0.0f) - c[5] * bm[k];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        const float cf = g2 * (eps[a] - vol * (1.0f / 3.0f));
+        for (int n = 3; n >= 0; --n) {
+            float t[9];
+            mul3f(bm, acc, t);
+#pragma unroll
+            for (int k = 0; k < 9; ++k) acc[k] = ((k == 0 || k == 4 || k == 8) ? c[n] : 0.0f) - t[k];
+        }
+        // acc = I - A/2 + A^2/3 - ...; ln(I + A) = A acc
+        float l[9];
+        mul3f(bm, acc, l);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 3; ++j) tau[3 * i + j] += cf * vec[3 * a + i] * vec[3 * a + j];
+            for (int j = 0; j < 3; ++j) eps[3 * i + j] = 0.25f * (l[3 * i + j] + l[3 * j + i]);
+    } else {
+        // large strains (rare): principal stretches in f64, where the
+        // smallest eigenvalues of b keep their relative accuracy
+        double bd[9], lam[3], vec[9];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const double h0 = hd[3 * i], h1 = hd[3 * i + 1], h2 = hd[3 * i + 2];
+                bd[3 * i + j] = (double)hd[3 * i + j] + (double)hd[3 * j + i] +
+                                (h0 * hd[3 * j] + h1 * hd[3 * j + 1] + h2 * hd[3 * j + 2]);
+            }
+        sym_eigen3<double>(bd, lam, vec);
+        double e3[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) e3[a] = 0.5 * log1p(fmax(lam[a], -1.0 + 1e-300));
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+                eps[3 * i + j] = (float)(e3[0] * vec[i] * vec[j] + e3[1] * vec[3 + i] * vec[3 + j] + e3[2] * vec[6 + i] * vec[6 + j]);
+    }
+    const float vol = eps[0] + eps[4] + eps[8];
+    const float pressure = mp.bulk * vol;
+    const float g2 = 2.0f * mp.shear;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const bool dg = (k == 0 || k == 4 || k == 8);
+        tau[k] = g2 * (eps[k] - (dg ? vol * (1.0f / 3.0f) : 0.0f)) + (dg ? pressure : 0.0f);
     }
 }
 
@@ -302,7 +380,7 @@ __device__ __forceinline__ void hencky_elastic(const MatParams& mp, const float*
 // dl[8][3]: displacements relative to node 0; g, gam, hs read by the
 // caller's accessors.
 template <class GradF, class GamF, class HF>
-__device__ __forceinline__ void element_nodal_forces(const float* tau, const float* F, float J, float vol,
+CRUSH_HD void element_nodal_forces(const float* tau, const float* F, float J, float vol,
                                                      const float (*dl)[3], GradF grad, GamF gam, HF hsym,
                                                      float (*fo)[3]) {
     float finv[9];
@@ -362,7 +440,7 @@ __device__ __forceinline__ void element_nodal_forces(const float* tau, const flo
 // diagonal splits with the 10 % barycentric tolerance.  Returns true and
 // the signed distance when accepted.  Compiled without FMA contraction so
 // the decision is the IEEE f64 result of the reference's expression.
-__device__ __forceinline__ bool point_in_tri(const double* p, const double* a, const double* b, const double* c) {
+CRUSH_HD bool point_in_tri(const double* p, const double* a, const double* b, const double* c) {
 #pragma clang fp contract(off)
     double v0[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
     double v1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
@@ -378,7 +456,7 @@ __device__ __forceinline__ bool point_in_tri(const double* p, const double* a, c
     return u >= -C_TRI_TOL && v >= -C_TRI_TOL && u + v <= 1.0 + C_TRI_TOL;
 }
 
-__device__ __forceinline__ bool face_accepts(const double* __restrict__ fg, int F, int f, const double* p,
+CRUSH_HD bool face_accepts(const double* __restrict__ fg, int F, int f, const double* p,
                                              double max_d, double* signed_out) {
 #pragma clang fp contract(off)
     auto at = [&](int k) { return fg[(size_t)k * F + f]; };
@@ -409,7 +487,7 @@ __device__ __forceinline__ bool face_accepts(const double* __restrict__ fg, int 
 // map, so the corners suffice); the node itself lies up to max_distance
 // behind that.  Ill-conditioned faces fall back to the inflated AABB, which
 // the acceptance test applies first anyway.
-__device__ __forceinline__ void face_geometry(const double (*x)[3], double max_d, double* out /* [FG] */) {
+CRUSH_HD void face_geometry(const double (*x)[3], double max_d, double* out /* [FG] */) {
 #pragma clang fp contract(off)
     double e1[3], e2[3], n[3];
 #pragma unroll
@@ -492,7 +570,7 @@ __device__ __forceinline__ void face_geometry(const double (*x)[3], double max_d
 }
 
 // V / A_max of a hexahedron from 8 positions (src/element.rs:176-186), f64.
-__device__ __forceinline__ void char_length_d(const double (*p)[3], double* length, double* volume) {
+CRUSH_HD void char_length_d(const double (*p)[3], double* length, double* volume) {
 #pragma clang fp contract(off)
     const int tets[6][4] = {{0, 1, 2, 6}, {0, 2, 3, 6}, {0, 3, 7, 6}, {0, 7, 4, 6}, {0, 4, 5, 6}, {0, 5, 1, 6}};
     const int faces[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4}, {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}};

@@ -79,9 +79,9 @@ struct DeviceModel {
         cls_dtf.upload(P.cls_dtf, s);
         elem_mass.upload(P.elem_mass, s);
         mats.upload(P.mats, s);
-        // honeycomb state: F_prev^-1 = I, R = I, sigma = 0, c = 0
+        // honeycomb state: H_prev = 0, R = I, sigma = 0, c = 0
         std::vector<float> st((size_t)25 * std::max(P.n_hc, 1), 0.0f);
-        for (int k : {0, 4, 8, 9, 13, 17})
+        for (int k : {9, 13, 17})
             for (int i = 0; i < P.n_hc; ++i) st[(size_t)k * P.n_hc + i] = 1.0f;
         state.upload(st, s);
         fe.alloc((size_t)24 * E);
@@ -283,25 +283,26 @@ __global__ __launch_bounds__(256) void k_contact_faces(int F, const int32_t* __r
 constexpr int GRID_MAX_CELLS = 2048;
 constexpr int GRID_MAX_ENTRIES = 8192;
 
-// Block-wide exclusive scan of s[0..n) in place (n <= 256 * 8), 256 threads.
-__device__ void block_exclusive_scan(int* s, int n, int* s_tmp /* [256] */) {
-    const int t = threadIdx.x;
+// Block-wide exclusive scan of s[0..n) in place, 256 threads (4 wave64s):
+// each thread scans a run of consecutive entries, the run totals are
+// scanned across each wave with shuffles and across waves through LDS.
+__device__ void block_exclusive_scan(int* s, int n, int* s_tmp /* [>= 4] */) {
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int per = (n + 255) / 256;
-    const int b = t * per, e = min(b + per, n);
+    const int b = min(t * per, n), e = min(b + per, n);
     int sum = 0;
     for (int i = b; i < e; ++i) sum += s[i];
-    s_tmp[t] = sum;
-    __syncthreads();
-    if (t == 0) {
-        int acc = 0;
-        for (int i = 0; i < 256; ++i) {
-            int v = s_tmp[i];
-            s_tmp[i] = acc;
-            acc += v;
-        }
+    int incl = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        int v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
     }
+    if (lane == 63) s_tmp[wave] = incl;
     __syncthreads();
-    int acc = s_tmp[t];
+    int base = 0;
+    for (int w = 0; w < wave; ++w) base += s_tmp[w];
+    int acc = base + incl - sum;
     for (int i = b; i < e; ++i) {
         int v = s[i];
         s[i] = acc;
@@ -660,7 +661,7 @@ __global__ __launch_bounds__(256) void k_force(int begin, int end, int E, int C,
         float st[25];
 #pragma unroll
         for (int k = 0; k < 25; ++k) st[k] = state[(size_t)k * n_hc + s];
-        honeycomb_update(mp, F, st, tau);
+        honeycomb_update(mp, hd, F, st, tau);
 #pragma unroll
         for (int k = 0; k < 25; ++k) state[(size_t)k * n_hc + s] = st[k];
     } else {
@@ -944,7 +945,8 @@ void run_plain(const crush_model& m, crush_results* out) {
     }
 }
 
-void run_plain_taps(const crush_model& m, int n_steps, double* f_int, double* f_ext, int32_t* hit, uint8_t* eroded,
+void run_plain_taps(const crush_model& m, int n_steps, double* f_int, double* f_ext, double* u_out, int32_t* hit,
+                    uint8_t* eroded,
                     double* dt, int32_t* steps_done) {
     Prepared P;
     prepare(m, P);
@@ -958,11 +960,12 @@ void run_plain_taps(const crush_model& m, int n_steps, double* f_int, double* f_
         if (before.done) break;
         S.launch_step();
         StepCtl c = S.read_ctl();
-        if (c.error) fail(CRUSH_E_NONFINITE, "displacement became non-finite at step %lld", c.step);
         if (f_int)
             HIP_OK(hipMemcpy(f_int + (size_t)s * 3 * N, S.D->f_int.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToHost));
         if (f_ext)
             HIP_OK(hipMemcpy(f_ext + (size_t)s * 3 * N, S.D->f_ext.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToHost));
+        if (u_out)
+            HIP_OK(hipMemcpy(u_out + (size_t)s * 3 * N, S.D->u.p, 3 * (size_t)N * sizeof(double), hipMemcpyDeviceToHost));
         if (hit && K > 0) {
             HIP_OK(hipMemcpy(rf.data(), S.D->rec_face.p, K * sizeof(int32_t), hipMemcpyDeviceToHost));
             for (int k = 0; k < K; ++k) {
@@ -976,6 +979,11 @@ void run_plain_taps(const crush_model& m, int n_steps, double* f_int, double* f_
         }
         if (dt) dt[s] = c.dt;
         done_steps = s + 1;
+        if (c.error) {
+            // the taps of the failing step are kept for inspection
+            if (steps_done) *steps_done = done_steps;
+            fail(CRUSH_E_NONFINITE, "displacement became non-finite at step %lld", c.step);
+        }
     }
     if (steps_done) *steps_done = done_steps;
 }

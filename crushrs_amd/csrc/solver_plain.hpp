@@ -11,7 +11,8 @@ namespace crush {
 void run_plain(const crush_model& m, crush_results* out);
 
 // First n_steps with per-step taps (crush_debug_run_steps).
-void run_plain_taps(const crush_model& m, int n_steps, double* f_int, double* f_ext, int32_t* hit, uint8_t* eroded,
+void run_plain_taps(const crush_model& m, int n_steps, double* f_int, double* f_ext, double* u, int32_t* hit,
+                    uint8_t* eroded,
                     double* dt, int32_t* steps_done);
 
 }  // namespace crush

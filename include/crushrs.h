@@ -175,6 +175,7 @@ int32_t crush_run(const crush_model* model, crush_results* out);
  * record, per step s (0-based, after the step), f_int[s*3N..] (internal
  * nodal forces at the new configuration, src/solver.rs:327-328),
  * f_ext[s*3N..] (contact forces used by that step, src/solver.rs:307-310),
+ * u[s*3N..] (displacement after the step),
  * hit[s*n_contact_nodes..] (face index hit by each contacting node of each
  * contact, -1 = none, contacts concatenated in order, nodes in contact
  * order after the static broad phase of src/contact.rs:48-70, i.e. only
@@ -182,7 +183,7 @@ int32_t crush_run(const crush_model* model, crush_results* out);
  * Any output pointer may be NULL.  Returns the number of steps taken in
  * *steps_done (fewer than n_steps if end_time was reached). */
 int32_t crush_debug_run_steps(const crush_model* model, int32_t n_steps,
-                              double* f_int, double* f_ext, int32_t* hit,
+                              double* f_int, double* f_ext, double* u, int32_t* hit,
                               uint8_t* eroded, double* dt, int32_t* steps_done);
 
 /* Number of contacting nodes the static broad phase keeps, summed over the

@@ -1,0 +1,88 @@
+"""The GPU kernels' f32 element arithmetic (compiled for the host) against
+the f64 oracle, element by element."""
+import numpy as np
+import pytest
+
+import host_kernels as hk
+import oracle_ffi as of
+from test_oracle_units import honeycomb_material, params, hc_update, virgin_state
+
+
+def f32(a):
+    return np.ascontiguousarray(a, np.float32)
+
+
+def hencky32(hd, e, nu):
+    tau = np.zeros(9, np.float32)
+    hk.load().hk_hencky(hk.fp(f32(hd).ravel()), e, nu, hk.fp(tau))
+    return tau.reshape(3, 3).astype(float)
+
+
+def hencky64(F, e, nu):
+    tau = np.zeros(9)
+    of.load().oracle_hencky_elastic(e, nu, of.ptr(np.ascontiguousarray(F, float).ravel()), of.ptr(tau))
+    return tau.reshape(3, 3)
+
+
+@pytest.mark.parametrize("scale", [1e-38, 1e-30, 1e-12, 1e-8, 1e-6, 1e-4, 1e-3, 1e-2, 0.02, 0.1, 0.3])
+def test_hencky_f32_matches_f64(scale):
+    rng = np.random.default_rng(int(-np.log10(scale) * 10))
+    e, nu = 3e8, 0.3
+    for _ in range(200):
+        hd = rng.normal(size=(3, 3)) * scale
+        hd32 = f32(hd).astype(float)
+        t32 = hencky32(hd32, e, nu)
+        assert np.isfinite(t32).all()
+        t64 = hencky64(np.eye(3) + hd32, e, nu)
+        n = np.linalg.norm(t64)
+        # f32 resolution of the strain: eps ~ 6e-8 relative to |Hd|, plus the
+        # oracle's own round-off of F F^T - I at tiny strains (~1e-16 E)
+        assert np.linalg.norm(t32 - t64) <= 2e-6 * n + 1e-15 * e, (scale, np.linalg.norm(t32 - t64) / max(n, 1e-300))
+
+
+def test_hencky_denormal_and_degenerate_inputs_stay_finite():
+    e, nu = 4.9e8, 0.3
+    cases = [np.zeros((3, 3)), np.diag([1e-40, 0, 0]), np.full((3, 3), 1e-39),
+             np.array([[0, 0, 0], [-6.6e-41, -1.4e-39, -6.1e-41], [1.5e-40, -2.2e-40, 1.5e-41]]),
+             np.eye(3) * 1e-3, np.eye(3) * 0.2]
+    for hd in cases:
+        assert np.isfinite(hencky32(hd, e, nu)).all()
+
+
+def test_honeycomb_f32_tracks_f64_under_crush_with_shear():
+    # the reference's lane-vs-scalar check (src/kernel_simd.rs:613-662): a
+    # 200-step crush with shear; compaction within 1e-3 relative, stresses
+    # within 1e-6 E + 2e-3 sigma_y(c)
+    for curve, dens, trans in [(None, None, 1.0),
+                               ([[0.0, 7000.0], [0.05, 10000.0], [0.15, 6000.0], [0.3, 30000.0], [0.45, 30000.0]],
+                                (1.0, 2.0e6), 1.0),
+                               ([[0.0, 7000.0], [0.05, 10000.0], [0.15, 6000.0], [0.3, 30000.0], [0.45, 30000.0]],
+                                (1.0, 2.0e6), 4.0)]:
+        if curve is None:
+            m = honeycomb_material(e=125.8e6, sy=1234.0, h=800339.0)
+        else:
+            m = honeycomb_material(e=23.1e6, curve=curve, dens=dens, trans=trans)
+        rc, knots, cap = params(m)
+        assert rc == 0
+        e = m.youngs_modulus
+        st64 = virgin_state()
+        st32 = np.zeros(25, np.float32)
+        st32[[9, 13, 17]] = 1.0
+        L = of.load()
+        for step in range(1, 201):
+            c = 0.5 * step / 200
+            hd = np.zeros((3, 3))
+            # the element of src/kernel_simd.rs:629-634: the +x face moves by
+            # (-c L, 0.02 c L), i.e. F = I + [[-c, 0, 0], [0.02 c, 0, 0], 0]
+            hd[0, 0] = -c
+            hd[1, 0] = 0.02 * c
+            tau64 = hc_update(e, knots, cap, np.eye(3) + hd, st64)
+            tau32 = np.zeros(9, np.float32)
+            hk.load().hk_honeycomb(hk.fp(f32(hd).ravel()), e, hk.fp(f32(knots)), trans, hk.fp(st32), hk.fp(tau32))
+            sy = np.zeros(1); h = np.zeros(1)
+            L.oracle_knot_yield_at(of.ptr(knots), st64[24], of.ptr(sy), of.ptr(h))
+            floor = 1e-6 * e + 2e-3 * sy[0]
+            assert abs(st32[24] - st64[24]) < 1e-3 * max(st64[24], 1e-6) + 1e-6, step
+            assert np.all(np.abs(st32[18:24] - st64[18:24]) < floor), step
+            assert np.all(np.abs(tau32 - tau64.ravel()) < floor * 1.0), step
+        assert st64[24] > 0.3

@@ -1,4 +1,10 @@
-"""GPU path vs the f64 oracle on the same models (parity tests proper)."""
+"""GPU path vs the f64 oracle on the same models (the parity tests proper).
+
+Bars: per step, nodal internal-force L2 relative
+error < 1e-4; contact-pair and erosion decisions identical for the first
+100 steps; per-vehicle delta-v within 0.05 m/s and crush within 2 mm at the
+end of the solve.
+"""
 import numpy as np
 import pytest
 
@@ -7,41 +13,140 @@ import oracle_ffi as of
 
 pytestmark = pytest.mark.gpu
 
+FORCE_RTOL = 1e-4
 
-def _step_parity(model, n_steps, force_tol=1e-4):
+
+def step_parity(model, n_steps):
     k = model.contact_node_count()
     gi, gx, gh, ge, gdt = ca.debug_run_steps(model, n_steps)
     oi, ox, oh, oe, odt = of.run_steps(model, n_steps, k)
     assert len(gdt) == len(odt) == n_steps
-    # erosion and contact-pair decisions identical step by step
-    assert np.array_equal(ge, oe)
+    assert np.array_equal(ge, oe), "erosion decisions differ"
     mism = np.argwhere(gh != oh)
     assert mism.size == 0, f"contact decisions differ at (step, node) {mism[:5].tolist()}"
-    # nodal internal-force L2 relative error per step
+    norms = np.linalg.norm(oi, axis=1)
+    peak = norms.max()
     for s in range(n_steps):
-        on = np.linalg.norm(oi[s])
-        if on == 0.0:
-            assert np.linalg.norm(gi[s]) < 1e-6
-            continue
-        rel = np.linalg.norm(gi[s] - oi[s]) / on
-        assert rel < force_tol, f"step {s}: f_int rel err {rel:.2e}"
+        err = np.linalg.norm(gi[s] - oi[s])
+        if norms[s] < 1e-3 * peak:
+            # rigid-motion steps: the oracle's f_int is round-off of a zero force
+            assert err < FORCE_RTOL * peak, f"step {s}: |df| {err:.3e} vs peak {peak:.3e}"
+        else:
+            assert err < FORCE_RTOL * norms[s], f"step {s}: f_int rel err {err / norms[s]:.2e}"
     np.testing.assert_allclose(gdt, odt, rtol=1e-5)
+    # contact forces follow from identical decisions at nearly equal states
+    xn = np.linalg.norm(ox, axis=1)
+    xpeak = max(xn.max(), 1e-30)
+    for s in range(n_steps):
+        assert np.linalg.norm(gx[s] - ox[s]) < 1e-3 * max(xn[s], 1e-3 * xpeak)
     return gi, oi
+
+
+def end_parity(model, dv_tol=0.05, crush_tol=2e-3):
+    g = ca.run(model)
+    o = of.run(model)
+    assert g.eroded_elements == o.eroded_elements
+    # the adaptive step follows the most crushed element, so long solves
+    # may differ by a few steps
+    assert abs(g.steps - o.steps) <= max(1, 0.01 * o.steps)
+    for a, b in zip(g.groups, o.groups):
+        assert np.all(np.abs(a.delta_v - b.delta_v) < dv_tol), (a.delta_v, b.delta_v)
+        assert abs(a.crush_end - b.crush_end) < crush_tol, (a.crush_end, b.crush_end)
+        assert abs(a.crush_max - b.crush_max) < crush_tol, (a.crush_max, b.crush_max)
+    return g, o
+
+
+def headon_toml(offset=0.3, h=0.3, end_time=0.15):
+    a = ca.renamed(ca.vehicle_file("neon_1996"), "neon_a")
+    b = ca.renamed(ca.vehicle_file("expedition_1999"), "expedition_b")
+    return ca.Model.headon(a, b, 25 * ca.MPH, 20 * ca.MPH, offset=offset, element_size=h, end_time=end_time)
 
 
 def test_crash_30_30_first_100_steps():
     print("case: crash 30 30, 100 steps", flush=True)
-    m = ca.Model.crash(30.0, 30.0)
-    _step_parity(m, 100)
+    step_parity(ca.Model.crash(30.0, 30.0), 100)
 
 
-def test_crash_30_30_delta_v_and_crush():
+def test_crash_30_30_delta_v_crush_and_momentum():
     print("case: crash 30 30 full solve", flush=True)
     m = ca.Model.crash(30.0, 30.0)
-    g = ca.run(m)
-    o = of.run(m)
-    assert g.eroded_elements == o.eroded_elements
-    for a, b in zip(g.groups, o.groups):
-        assert abs(a.delta_v[0] - b.delta_v[0]) < 0.05, (a.delta_v, b.delta_v)
-        assert abs(a.crush_end - b.crush_end) < 2e-3
-        assert abs(a.crush_max - b.crush_max) < 2e-3
+    g, o = end_parity(m)
+    # momentum exchanged only through contact (tests/impact.rs:37-38)
+    mt, dvt = g.groups[0].mass, g.groups[0].delta_v[0]
+    mc, dvc = g.groups[1].mass, g.groups[1].delta_v[0]
+    assert abs(mt * dvt + mc * dvc) < 1e-6 * mt * 30.0 * ca.MPH
+    # plausibility bounds of tests/impact.rs:39-46
+    v0 = 30.0 * ca.MPH
+    plastic = (mt * v0 - mc * v0) / (mt + mc) + v0
+    assert plastic < dvc < 1.3 * plastic
+    e = ((dvc - v0) - (v0 + dvt)) / (2 * v0)
+    assert 0.0 < e < 0.3
+    assert g.eroded_elements == 0
+
+
+def test_headon_toml_offset_road_first_100_steps():
+    print("case: neon_1996 vs expedition_1999, offset 0.3, road, 100 steps", flush=True)
+    step_parity(headon_toml(), 100)
+
+
+def test_headon_toml_offset_road_full():
+    print("case: neon_1996 vs expedition_1999, offset 0.3, road, full solve", flush=True)
+    end_parity(headon_toml())
+
+
+def test_tbone_first_100_steps_and_full():
+    print("case: tbone silverado into neon side, road", flush=True)
+    m = ca.Model.tbone(ca.vehicle("silverado"), ca.vehicle("neon"), 30 * ca.MPH, 0.0, offset=-0.4)
+    step_parity(m, 100)
+    end_parity(m)
+
+
+def test_barrier_pulse_vehicle():
+    print("case: barrier silverado_2007 35 mph (wall + road), 0.05 s", flush=True)
+    m = ca.Model.barrier(ca.vehicle_file("silverado_2007"), end_time=0.05)
+    step_parity(m, 100)
+    end_parity(m)
+
+
+def single_hex(v_top=-250.0):
+    """One honeycomb cube, bottom fixed, top driven through the bottom."""
+    from crushrs_amd import _abi
+    import ctypes as C
+    xi = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                   [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], float)
+    nodes = np.ascontiguousarray(0.15 * (xi + 1.0))
+    hexes = np.arange(8, dtype=np.int32)
+    part = np.zeros(1, np.int32)
+    iv = np.zeros((8, 3)); iv[4:, 2] = v_top
+    fixed = np.arange(4, dtype=np.int32)
+    mat = (_abi.Material * 1)()
+    mat[0].youngs_modulus, mat[0].density, mat[0].model = 20e6, 200.0, _abi.MAT_HONEYCOMB
+    mat[0].yield_stress, mat[0].hardening, mat[0].transverse_factor = 0.2e6, 0.5e6, 1.0
+    mv = _abi.ModelView()
+    mv.n_nodes, mv.n_hexes, mv.n_parts, mv.n_fixed = 8, 1, 1, 4
+    keep = [nodes, hexes, part, iv, fixed, mat]
+    mv.nodes = nodes.ctypes.data_as(_abi.f64p)
+    mv.hexes = hexes.ctypes.data_as(_abi.i32p)
+    mv.hex_part = part.ctypes.data_as(_abi.i32p)
+    mv.materials = C.cast(mat, C.POINTER(_abi.Material))
+    mv.initial_velocity = iv.ctypes.data_as(_abi.f64p)
+    mv.fixed_nodes = fixed.ctypes.data_as(_abi.i32p)
+    mv.settings = ca.default_settings()
+    mv.settings.end_time = 0.01
+    return mv, keep
+
+
+def test_erosion_decisions_match():
+    print("case: single hex driven to inversion", flush=True)
+    import ctypes as C
+    mv, keep = single_hex()
+    lib = ca.load()
+    n = 60
+    gi = np.zeros((n, 24)); ge = np.zeros((n, 1), np.uint8); gd = np.zeros(n); done = C.c_int32()
+    ca.check(lib.crush_debug_run_steps(C.byref(mv), n, of.ptr(gi), None, None, None, of.ptr(ge), of.ptr(gd), C.byref(done)))
+    oi = np.zeros((n, 24)); oe = np.zeros((n, 1), np.uint8); od = np.zeros(n); done2 = C.c_int32()
+    assert of.load().oracle_run_steps(C.byref(mv), n, of.ptr(oi), None, None, of.ptr(oe), of.ptr(od), C.byref(done2)) == 0
+    assert done.value == done2.value
+    s = done.value
+    assert oe[:s].any(), "the case must erode"
+    assert np.array_equal(ge[:s], oe[:s])
