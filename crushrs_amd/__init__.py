@@ -266,9 +266,59 @@ def debug_run_steps(model: Model, n_steps: int):
     return fi[:s], fx[:s], hit[:s, :k], er[:s], dt[:s]
 
 
+def batch_options(**kw) -> _abi.BatchOptions:
+    """crush_batch_defaults with fields overridden by keyword."""
+    o = _abi.BatchOptions()
+    load().crush_batch_defaults(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Batch:
+    """Many independent models solved together on the GPU (the reference's
+    par_iter over solver::run, src/vehicle.rs:1227-1233).  Results of each
+    instance match crush_run on that model alone."""
+
+    def __init__(self, models, options: Optional[_abi.BatchOptions] = None, **kw):
+        self._lib = load()
+        self.models = list(models)
+        self._n_nodes = [m.n_nodes for m in self.models]
+        self._n_groups = [m.view.n_groups for m in self.models]
+        arr = (C.POINTER(_abi.ModelView) * len(self.models))(*[m.ptr for m in self.models])
+        opt = options if options is not None else batch_options(**kw)
+        self._h = C.c_void_p()
+        check(self._lib.crush_batch_create(arr, len(self.models), C.byref(opt), C.byref(self._h)))
+        self.stats = _abi.BatchStats()
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.crush_batch_free(h)
+            self._h = None
+
+    def __len__(self) -> int:
+        return len(self.models)
+
+    def run(self) -> _abi.BatchStats:
+        check(self._lib.crush_batch_run(self._h, C.byref(self.stats)))
+        return self.stats
+
+    def reset(self) -> None:
+        check(self._lib.crush_batch_reset(self._h))
+
+    def result(self, i: int, fields: bool = True) -> Results:
+        r, u, v, m, g = alloc_results(self._n_nodes[i], self._n_groups[i])
+        if not fields:
+            r.displacement = None
+            r.velocity = None
+        check(self._lib.crush_batch_result(self._h, i, C.byref(r)))
+        return to_results(r, u, v, m, g)
+
+
 def device_count() -> int:
     return int(load().crush_device_count())
 
 
 __all__ = ["MPH", "Model", "Results", "run", "vehicle", "vehicle_file", "side_profile", "rear_profile",
-           "default_settings", "debug_run_steps", "device_count", "CrushError", "renamed"]
+           "default_settings", "debug_run_steps", "device_count", "CrushError", "renamed", "Batch", "batch_options"]

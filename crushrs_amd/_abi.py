@@ -172,6 +172,30 @@ class HeadOnOptions(C.Structure):
     ]
 
 
+class BatchOptions(C.Structure):
+    _fields_ = [
+        ("threads_per_instance", C.c_int32),
+        ("steps_per_launch", C.c_int32),
+        ("launches_per_graph", C.c_int32),
+        ("host_threads", C.c_int32),
+        ("contact_cell_scale", C.c_double),
+    ]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("prepare_ms", C.c_double),
+        ("upload_ms", C.c_double),
+        ("device_ms", C.c_double),
+        ("wall_ms", C.c_double),
+        ("instances", C.c_int64),
+        ("element_steps", C.c_int64),
+        ("launches", C.c_int64),
+        ("graph_replays", C.c_int64),
+        ("pool_bytes", C.c_double),
+    ]
+
+
 LIB_PATH = Path(__file__).resolve().parent / "libcrushrs_amd.so"
 
 # Every symbol the public headers declare, with (restype, argtypes).
@@ -182,6 +206,13 @@ SIGNATURES = {
     "crush_run": (C.c_int32, [C.POINTER(ModelView), C.POINTER(Results)]),
     "crush_debug_run_steps": (C.c_int32, [C.POINTER(ModelView), C.c_int32, f64p, f64p, f64p, i32p, u8p, f64p, i32p]),
     "crush_contact_node_count": (C.c_int32, [C.POINTER(ModelView), i32p]),
+    "crush_batch_defaults": (None, [C.POINTER(BatchOptions)]),
+    "crush_batch_create": (C.c_int32, [C.POINTER(C.POINTER(ModelView)), C.c_int32, C.POINTER(BatchOptions),
+                                       C.POINTER(C.c_void_p)]),
+    "crush_batch_run": (C.c_int32, [C.c_void_p, C.POINTER(BatchStats)]),
+    "crush_batch_reset": (C.c_int32, [C.c_void_p]),
+    "crush_batch_result": (C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(Results)]),
+    "crush_batch_free": (None, [C.c_void_p]),
     "crush_headon_defaults": (None, [C.POINTER(HeadOnOptions)]),
     "crush_vehicle_load": (C.c_int32, [C.c_char_p, C.POINTER(Vehicle)]),
     "crush_vehicle_side_profile": (C.c_int32, [C.POINTER(Vehicle), C.POINTER(Vehicle)]),

@@ -36,6 +36,7 @@ struct StepCtl {
     int quiet, separated, pair_hits, chk;
     int sep_steps, adaptive, reserved0, reserved1;
     double crush_max[8];
+    double crush_end[8];
 };
 
 // Face geometry record width (doubles) in the SoA face array:
@@ -88,8 +89,7 @@ CRUSH_HD void mul3f(const float* a, const float* b, float* r) {
 // (H - H_prev) F_prev^-1, the same quantity without the cancellation
 // against I that limits an f32 F_prev^-1.  Updates st and returns the
 // Kirchhoff stress tau (row-major).
-CRUSH_HD void honeycomb_update(const MatParams& mp, const float* hd, const float* F, float* st,
-                                                 float* tau) {
+CRUSH_HD void honeycomb_update(const MatParams& mp, const float* hd, float* st, float* tau) {
     float fprev[9], fpinv[9], dh[9], mid[9], midinv[9], g[9], dinc[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
@@ -140,15 +140,13 @@ CRUSH_HD void honeycomb_update(const MatParams& mp, const float* hd, const float
     s[4] = st[22] + e * de[5];
     s[5] = st[23] + e * de[2];
     const float c = st[24];
-    // knot scan (unused knots sit at +inf)
-    float kc = mp.knot_c[0], ks = mp.knot_s[0], kh = mp.knot_h[0];
+    // Active knot: the last i with c >= knot_c[i] (src/material.rs:459-467);
+    // knots strictly increase with unused ones at +inf, so it is the count
+    // of knots i >= 1 at or below c.
+    int ki = 0;
 #pragma unroll
-    for (int i = 1; i < 8; ++i) {
-        bool m = c >= mp.knot_c[i];
-        kc = m ? mp.knot_c[i] : kc;
-        ks = m ? mp.knot_s[i] : ks;
-        kh = m ? mp.knot_h[i] : kh;
-    }
+    for (int i = 1; i < 8; ++i) ki += (c >= mp.knot_c[i]) ? 1 : 0;
+    const float kc = mp.knot_c[ki], ks = mp.knot_s[ki], kh = mp.knot_h[ki];
     const float hard = kh;
     const float syc = ks + hard * (c - kc);
     const float tk[3] = {1.0f, mp.trans, mp.trans};
@@ -177,6 +175,9 @@ CRUSH_HD void honeycomb_update(const MatParams& mp, const float* hd, const float
     const float sh[9] = {s[0], s[3], s[5], s[3], s[1], s[4], s[5], s[4], s[2]};
     float rs[9];
     mul3f(r, sh, rs);
+    float F[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) F[k] = hd[k] + ((k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f);
     const float J = det3f(F);
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -304,6 +305,31 @@ CRUSH_HD void sym_eigen3(const T* a, T* vals, T* vecs) {
 
 CRUSH_HD void sym_eigen3f(const float* a, float* vals, float* vecs) { sym_eigen3<float>(a, vals, vecs); }
 
+// Large-strain Hencky strain eps = ln(F F^T)/2 from principal stretches in
+// f64, where the smallest eigenvalues of b keep their relative accuracy.
+// Kept out of line: rare, and its f64 temporaries would otherwise raise the
+// register budget of every caller.
+__host__ __device__ __attribute__((noinline)) inline void hencky_log_strain_large(const float* hd, float* eps) {
+    double bd[9], lam[3], vec[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const double h0 = hd[3 * i], h1 = hd[3 * i + 1], h2 = hd[3 * i + 2];
+            bd[3 * i + j] = (double)hd[3 * i + j] + (double)hd[3 * j + i] +
+                            (h0 * hd[3 * j] + h1 * hd[3 * j + 1] + h2 * hd[3 * j + 2]);
+        }
+    sym_eigen3<double>(bd, lam, vec);
+    double e3[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) e3[a] = 0.5 * log1p(fmax(lam[a], -1.0 + 1e-300));
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            eps[3 * i + j] = (float)(e3[0] * vec[i] * vec[j] + e3[1] * vec[3 + i] * vec[3 + j] + e3[2] * vec[6 + i] * vec[6 + j]);
+}
+
 // Hencky elastic Kirchhoff stress (src/material.rs:708-771 with no
 // plasticity): tau = K tr(eps) I + 2G dev(eps) with eps = ln(b)/2, b = F F^T.
 // The small quantity A = b - I = Hd + Hd^T + Hd Hd^T is formed from the
@@ -344,26 +370,7 @@ CRUSH_HD void hencky_elastic(const MatParams& mp, const float* hd, float* tau) {
 #pragma unroll
             for (int j = 0; j < 3; ++j) eps[3 * i + j] = 0.25f * (l[3 * i + j] + l[3 * j + i]);
     } else {
-        // large strains (rare): principal stretches in f64, where the
-        // smallest eigenvalues of b keep their relative accuracy
-        double bd[9], lam[3], vec[9];
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const double h0 = hd[3 * i], h1 = hd[3 * i + 1], h2 = hd[3 * i + 2];
-                bd[3 * i + j] = (double)hd[3 * i + j] + (double)hd[3 * j + i] +
-                                (h0 * hd[3 * j] + h1 * hd[3 * j + 1] + h2 * hd[3 * j + 2]);
-            }
-        sym_eigen3<double>(bd, lam, vec);
-        double e3[3];
-#pragma unroll
-        for (int a = 0; a < 3; ++a) e3[a] = 0.5 * log1p(fmax(lam[a], -1.0 + 1e-300));
-#pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int j = 0; j < 3; ++j)
-                eps[3 * i + j] = (float)(e3[0] * vec[i] * vec[j] + e3[1] * vec[3 + i] * vec[3 + j] + e3[2] * vec[6 + i] * vec[6 + j]);
+        hencky_log_strain_large(hd, eps);
     }
     const float vol = eps[0] + eps[4] + eps[8];
     const float pressure = mp.bulk * vol;

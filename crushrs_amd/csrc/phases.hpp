@@ -1,0 +1,658 @@
+// Per-item device work of one explicit step, shared by the one-solve
+// kernels (solver_plain.hip) and the batched persistent kernel (batch.hip):
+// each function does the work of one element, node, face or contacting node.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device_common.hpp"
+
+namespace crush {
+
+// Raw buffer access: one SGPR resource per array and 32-bit offsets, so a
+// loop over uniform strides (SoA rows, class constants) shares one VGPR
+// offset instead of materialising a 64-bit address per load.  Offsets past
+// the resource's size read as zero and drop writes.
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ rsrc_t make_rsrc(const void* p, size_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, (int)(unsigned)bytes, 0x00020000);
+}
+__device__ __forceinline__ float ld_f32(rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ int ld_i32(rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(int, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ double ld_f64(rsrc_t r, unsigned voff, unsigned soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ void st_f32(rsrc_t r, float v, unsigned voff, unsigned soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
+}
+
+// Everything a step touches for one model (one instance), as device
+// pointers.  The one-solve path fills it from its buffers; the batched path
+// keeps one per instance in a table.
+struct InstDev {
+    int N, E, n_hc, C, F, K, T, n_contacts, n_groups, n_sblk;
+    // elements
+    const int32_t* conn;
+    const int32_t* ecls;
+    const int32_t* cls_mat;
+    const float* cgrad;
+    const float* cvol;
+    const float* cgam;
+    const float* chs;
+    const double* cdtf;
+    const double* emass;
+    const MatParams* mats;
+    float* state;
+    uint8_t* eroded;
+    float* fe;
+    double* extra;
+    // nodes
+    const double* X;
+    double* u;
+    double* v;
+    double* f_int;
+    double* f_ext;
+    double* inv_mass;
+    const double* mass;
+    double* added;
+    const uint8_t* fixed;
+    const int32_t* n2e_ptr;
+    const int32_t* n2e_idx;
+    // contacts
+    const ContactDesc* cdesc;
+    const int32_t* cn_node;
+    const double* cn_scale;
+    const int32_t* cf_nodes;
+    const int32_t* cf_contact;
+    const int32_t* tn_node;
+    const int32_t* tn_ptr;
+    const int32_t* tn_ent;
+    const int32_t* sblk_contact;
+    const int32_t* sblk_start;
+    double* fgeo;
+    int32_t* rec_face;
+    double* rec_f;
+    int32_t* fh_count;
+    int32_t* fh_list;
+    double* face_force;
+    // groups
+    const int32_t* g_nodes_ptr;
+    const int32_t* g_nodes;
+    const int32_t* g_front_ptr;
+    const int32_t* g_front;
+    const int32_t* g_accel;
+    const int32_t* g_axis;
+    const double* g_mass;
+    const double* g_v0;   // [3G] initial mean velocities
+    double* g_dv;         // [3G] delta-v written when the solve ends
+    StepCtl* ctl;
+};
+
+__device__ __forceinline__ double step_dt(const StepCtl& c) { return fmax(fmin(c.dt_cfl, c.end_time - c.t), 1e-15); }
+
+// Stable step and scaling mass of one element (src/kernel.rs:214-258).
+__device__ __forceinline__ void dt_element(const InstDev& D, int s, double dt_floor, double* dte_out, double* ex_out) {
+#pragma clang fp contract(off)
+    double dte = INFINITY, ex = 0.0;
+    if (!D.eroded[s]) {
+        double p[8][3];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int n = D.conn[(size_t)i * D.E + s];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) p[i][d] = D.X[3 * (size_t)n + d] + D.u[3 * (size_t)n + d];
+        }
+        double l, vol;
+        char_length_d(p, &l, &vol);
+        if (isfinite(l) && vol > 0.0) dte = l * D.cdtf[D.ecls[s]];
+    }
+    if (dte < dt_floor) {
+        const double r = dt_floor / dte;
+        ex = (fmin(r * r, 100.0) - 1.0) * D.emass[s];
+    }
+    D.extra[s] = ex;
+    *dte_out = dte;
+    *ex_out = ex;
+}
+
+// Added mass and inverse mass of one node.
+__device__ __forceinline__ void mass_node(const InstDev& D, int n) {
+#pragma clang fp contract(off)
+    double a = 0.0;
+    for (int k = D.n2e_ptr[n]; k < D.n2e_ptr[n + 1]; ++k) a += D.extra[D.n2e_idx[k] >> 3] / 8.0;
+    D.added[n] = a;
+    const double m = D.mass[n] + a;
+    D.inv_mass[n] = m > 0.0 ? 1.0 / m : 0.0;
+}
+
+// Geometry of contact face f for the current configuration.
+__device__ __forceinline__ void contact_face(const InstDev& D, int f) {
+#pragma clang fp contract(off)
+    double x[4][3];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int n = D.cf_nodes[(size_t)k * D.F + f];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) x[k][d] = D.X[3 * (size_t)n + d] + D.u[3 * (size_t)n + d];
+    }
+    double g[FG];
+    face_geometry(x, D.cdesc[D.cf_contact[f]].max_distance, g);
+#pragma unroll
+    for (int k = 0; k < FG; ++k) D.fgeo[(size_t)k * D.F + f] = g[k];
+    D.fh_count[f] = 0;
+}
+
+// Grid over one contact's face candidate boxes (built in LDS).
+struct FaceGrid {
+    double glo[3];
+    double h;
+    int dim[3];
+    int overflow;
+};
+
+__device__ __forceinline__ int grid_cell(const FaceGrid& g, double x, int d) {
+    int ci = (int)floor((x - g.glo[d]) / g.h);
+    return min(max(ci, 0), g.dim[d] - 1);
+}
+
+// Penalty force of contacting node gk (src/contact.rs:113-157): the
+// lowest-index accepting face among the grid candidates (or, on grid
+// overflow, the first accepting face in order).  Records the hit and
+// returns 1 when the node penetrates.
+__device__ __forceinline__ int contact_node(const InstDev& D, const ContactDesc& cd, int k, const FaceGrid& g,
+                                            const int* s_start, const int* s_entries, double dt) {
+#pragma clang fp contract(off)
+    const int gk = cd.node_begin + k;
+    const int node = D.cn_node[gk];
+    const int F = D.F, fb = cd.face_begin;
+    double p[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) p[d] = D.X[3 * (size_t)node + d] + D.u[3 * (size_t)node + d];
+    int best = -1;
+    double best_sd = 0.0;
+    if (g.overflow) {
+        for (int f = 0; f < cd.n_faces; ++f) {
+            double sd;
+            if (face_accepts(D.fgeo, F, fb + f, p, cd.max_distance, &sd)) {
+                best = f;
+                best_sd = sd;
+                break;
+            }
+        }
+    } else {
+        bool inside = true;
+        int ci[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double r = (p[d] - g.glo[d]) / g.h;
+            inside = inside && r >= 0.0 && r < (double)g.dim[d] + 1.0;
+            ci[d] = min(max((int)floor(r), 0), g.dim[d] - 1);
+        }
+        if (inside) {
+            const int cell = (ci[2] * g.dim[1] + ci[1]) * g.dim[0] + ci[0];
+            for (int q = s_start[cell]; q < s_start[cell + 1]; ++q) {
+                const int f = s_entries[q];
+                if (best >= 0 && f >= best) continue;
+                double sd;
+                if (face_accepts(D.fgeo, F, fb + f, p, cd.max_distance, &sd)) {
+                    best = f;
+                    best_sd = sd;
+                }
+            }
+        }
+    }
+    if (best < 0) {
+        D.rec_face[gk] = -1;
+        return 0;
+    }
+    const int gf = fb + best;
+    const double m = D.mass[node];
+    const double depth = fmin(-best_sd, cd.depth_cap);
+    const double kk = fmin(cd.stiffness * D.cn_scale[gk], C_SOFT * m / (dt * dt));
+    const double mag = fmin(kk * depth + 0.0, m * C_MAX_ACCEL);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) D.rec_f[3 * (size_t)gk + d] = mag * D.fgeo[(size_t)(FG_N + d) * F + gf];
+    D.rec_face[gk] = gf;
+    const int slot = atomicAdd(&D.fh_count[gf], 1);
+    if (slot < FACE_HITS_CAP) D.fh_list[(size_t)gf * FACE_HITS_CAP + slot] = gk;
+    return 1;
+}
+
+// Sum of the quarter reactions face f receives, in contacting-node order.
+__device__ __forceinline__ void contact_face_force(const InstDev& D, int f) {
+#pragma clang fp contract(off)
+    const int cnt = D.fh_count[f];
+    double s[3] = {0.0, 0.0, 0.0};
+    if (cnt > 0 && cnt <= FACE_HITS_CAP) {
+        int ids[FACE_HITS_CAP];
+        for (int i = 0; i < cnt; ++i) ids[i] = D.fh_list[(size_t)f * FACE_HITS_CAP + i];
+        for (int i = 1; i < cnt; ++i) {
+            int x = ids[i], j = i - 1;
+            while (j >= 0 && ids[j] > x) {
+                ids[j + 1] = ids[j];
+                --j;
+            }
+            ids[j + 1] = x;
+        }
+        for (int i = 0; i < cnt; ++i)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) s[d] += 0.25 * D.rec_f[3 * (size_t)ids[i] + d];
+    } else if (cnt > FACE_HITS_CAP) {
+        const ContactDesc cd = D.cdesc[D.cf_contact[f]];
+        for (int k = cd.node_begin; k < cd.node_begin + cd.n_nodes; ++k)
+            if (D.rec_face[k] == f)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) s[d] += 0.25 * D.rec_f[3 * (size_t)k + d];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) D.face_force[3 * (size_t)f + d] = s[d];
+}
+
+// External (contact) force of touched node i, in contact order.
+__device__ __forceinline__ void contact_apply_node(const InstDev& D, int i) {
+#pragma clang fp contract(off)
+    double s[3] = {0.0, 0.0, 0.0};
+    for (int k = D.tn_ptr[i]; k < D.tn_ptr[i + 1]; ++k) {
+        const int e = D.tn_ent[k];
+        if (e >= 0) {
+            if (D.rec_face[e] >= 0)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) s[d] += D.rec_f[3 * (size_t)e + d];
+        } else {
+            const int f = -e - 1;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) s[d] -= D.face_force[3 * (size_t)f + d];
+        }
+    }
+    const int n = D.tn_node[i];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) D.f_ext[3 * (size_t)n + d] = s[d];
+}
+
+// First half step of node n (src/solver.rs:313-323); v holds v_1/2 after.
+__device__ __forceinline__ void half_node(const InstDev& D, int n, double dt) {
+#pragma clang fp contract(off)
+    if (D.fixed[n]) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            D.u[3 * (size_t)n + d] = 0.0;
+            D.v[3 * (size_t)n + d] = 0.0;
+        }
+        return;
+    }
+    const double im = D.inv_mass[n];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const size_t k = 3 * (size_t)n + d;
+        const double a = (D.f_ext[k] - D.f_int[k]) * im;
+        const double vh = D.v[k] + 0.5 * dt * a;
+        D.v[k] = vh;
+        D.u[k] += dt * vh;
+    }
+}
+
+// Element force of slot s (f32), streamed so that few values are live at
+// once: (1) one pass over the nodes accumulates the displacement gradient
+// H = sum_i (u_i - u_0) (x) grad_i and the hourglass coordinates
+// q = sum_i (u_i - u_0) gamma_i; (2) r = H_hg q; (3) the material update
+// gives tau; (4) each node's force tau F^-T grad_i V + r gamma_i^T is
+// written as it is formed.  Returns 1 when the element inverts now.
+template <int KIND>
+__device__ __forceinline__ int force_element(const InstDev& D, int s) {
+    const unsigned E = (unsigned)D.E, C = (unsigned)D.C, nh = (unsigned)D.n_hc;
+    const rsrc_t rfe = make_rsrc(D.fe, (size_t)24 * E * 4);
+    const unsigned vs = (unsigned)s * 4u;
+    if (D.eroded[s]) {
+#pragma unroll
+        for (int k = 0; k < 24; ++k) st_f32(rfe, 0.0f, vs, k * E * 4u);
+        return 0;
+    }
+    const rsrc_t rconn = make_rsrc(D.conn, (size_t)8 * E * 4);
+    const rsrc_t ru = make_rsrc(D.u, (size_t)D.N * 24);
+    const rsrc_t rgrad = make_rsrc(D.cgrad, (size_t)24 * C * 4);
+    const rsrc_t rgam = make_rsrc(D.cgam, (size_t)32 * C * 4);
+    const rsrc_t rhs = make_rsrc(D.chs, (size_t)HSYM * C * 4);
+    const int c = D.ecls[s];
+    const unsigned vc = (unsigned)c * 4u;
+    const unsigned vn0 = (unsigned)ld_i32(rconn, vs, 0) * 24u;
+    const double u0x = ld_f64(ru, vn0, 0), u0y = ld_f64(ru, vn0 + 8u, 0), u0z = ld_f64(ru, vn0 + 16u, 0);
+    float hd[9], q[12];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) hd[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 12; ++k) q[k] = 0.0f;
+#pragma unroll
+    for (int i = 1; i < 8; ++i) {
+        const unsigned vn = (unsigned)ld_i32(rconn, vs, i * E * 4u) * 24u;
+        const float d0 = (float)(ld_f64(ru, vn, 0) - u0x);
+        const float d1 = (float)(ld_f64(ru, vn + 8u, 0) - u0y);
+        const float d2 = (float)(ld_f64(ru, vn + 16u, 0) - u0z);
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const float g = ld_f32(rgrad, vc, (3 * i + b) * C * 4u);
+            hd[b] += d0 * g;
+            hd[3 + b] += d1 * g;
+            hd[6 + b] += d2 * g;
+        }
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const float ga = ld_f32(rgam, vc, (4 * i + a) * C * 4u);
+            q[3 * a] += d0 * ga;
+            q[3 * a + 1] += d1 * ga;
+            q[3 * a + 2] += d2 * ga;
+        }
+    }
+    float F[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) F[k] = hd[k] + ((k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f);
+    const float J = det3f(F);
+    if (!(J > 0.0f)) {
+        D.eroded[s] = 1;
+#pragma unroll
+        for (int k = 0; k < 24; ++k) st_f32(rfe, 0.0f, vs, k * E * 4u);
+        return 1;
+    }
+    // r = H q with H packed as its upper triangle
+    float rr[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) rr[k] = 0.0f;
+    {
+        int kk = 0;
+#pragma unroll
+        for (int r = 0; r < 12; ++r)
+#pragma unroll
+            for (int cc = r; cc < 12; ++cc) {
+                const float h = ld_f32(rhs, vc, (unsigned)(kk++) * C * 4u);
+                rr[r] += h * q[cc];
+                if (cc != r) rr[cc] += h * q[r];
+            }
+    }
+    const MatParams& mp = D.mats[D.cls_mat[c]];
+    float tau[9];
+    if (KIND == CRUSH_MAT_HONEYCOMB) {
+        const rsrc_t rst = make_rsrc(D.state, (size_t)25 * nh * 4);
+        float st[25];
+#pragma unroll
+        for (int k = 0; k < 25; ++k) st[k] = ld_f32(rst, vs, k * nh * 4u);
+        honeycomb_update(mp, hd, st, tau);
+#pragma unroll
+        for (int k = 0; k < 25; ++k) st_f32(rst, st[k], vs, k * nh * 4u);
+    } else {
+        hencky_elastic(mp, hd, tau);
+    }
+    // M = tau F^-T V
+    float finv[9], M[9];
+    inv3f(F, J, finv);
+    const float vol = D.cvol[c];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            M[3 * i + j] = vol * (tau[3 * i] * finv[3 * j] + tau[3 * i + 1] * finv[3 * j + 1] + tau[3 * i + 2] * finv[3 * j + 2]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const float g0 = ld_f32(rgrad, vc, (3 * i) * C * 4u);
+        const float g1 = ld_f32(rgrad, vc, (3 * i + 1) * C * 4u);
+        const float g2 = ld_f32(rgrad, vc, (3 * i + 2) * C * 4u);
+        float ga[4];
+#pragma unroll
+        for (int a = 0; a < 4; ++a) ga[a] = ld_f32(rgam, vc, (4 * i + a) * C * 4u);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            float f = M[3 * d] * g0 + M[3 * d + 1] * g1 + M[3 * d + 2] * g2;
+#pragma unroll
+            for (int a = 0; a < 4; ++a) f += rr[3 * a + d] * ga[a];
+            st_f32(rfe, f, vs, (3 * i + d) * E * 4u);
+        }
+    }
+    return 0;
+}
+
+// Node-centric gather of the element forces (f64 sums in element order)
+// and the second half step (src/solver.rs:327-342).  Returns false when the
+// node's state is not finite.
+__device__ __forceinline__ bool gather_node(const InstDev& D, int n, double dt, double damp) {
+#pragma clang fp contract(off)
+    double f[3] = {0.0, 0.0, 0.0};
+    const int E = D.E;
+    for (int k = D.n2e_ptr[n]; k < D.n2e_ptr[n + 1]; ++k) {
+        const int code = D.n2e_idx[k];
+        const size_t base = (size_t)(3 * (code & 7)) * E + (code >> 3);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) f[d] += (double)D.fe[base + (size_t)d * E];
+    }
+    const double im = D.inv_mass[n];
+    const bool fx = D.fixed[n] != 0;
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const size_t k = 3 * (size_t)n + d;
+        D.f_int[k] = f[d];
+        const double a = (D.f_ext[k] - f[d]) * im;
+        double vn = (D.v[k] + 0.5 * dt * a) * damp;
+        if (fx) vn = 0.0;
+        D.v[k] = vn;
+        ok = ok && isfinite(vn) && isfinite(D.u[k]);
+    }
+    return ok;
+}
+
+}  // namespace crush
+
+namespace crush {
+
+// ---------------------------------------------------------- block helpers
+
+template <int B>
+__device__ __forceinline__ double wave_reduce(double v, int op /* 0 sum, 1 min, 2 max */) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(v, off, 64);
+        v = op == 0 ? v + o : (op == 1 ? fmin(v, o) : fmax(v, o));
+    }
+    return v;
+}
+
+// Block-wide reduction with a fixed order (wave trees, then waves in
+// order); every thread gets the result.  s_w holds B/64 doubles.
+template <int B>
+__device__ __forceinline__ double block_reduce(double v, int op, double* s_w) {
+    v = wave_reduce<B>(v, op);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) s_w[wave] = v;
+    __syncthreads();
+    double r = s_w[0];
+    for (int w = 1; w < B / 64; ++w) r = op == 0 ? r + s_w[w] : (op == 1 ? fmin(r, s_w[w]) : fmax(r, s_w[w]));
+    __syncthreads();
+    return r;
+}
+
+// Block-wide exclusive scan of s[0..n) in place (B threads): runs of
+// consecutive entries per thread, shuffled scan across each wave, waves
+// in order through s_tmp (B/64 ints).
+template <int B>
+__device__ void block_scan(int* s, int n, int* s_tmp) {
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int per = (n + B - 1) / B;
+    const int b = min(t * per, n), e = min(b + per, n);
+    int sum = 0;
+    for (int i = b; i < e; ++i) sum += s[i];
+    int incl = sum;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int v = __shfl_up(incl, off, 64);
+        if (lane >= off) incl += v;
+    }
+    if (lane == 63) s_tmp[wave] = incl;
+    __syncthreads();
+    int acc = incl - sum;
+    for (int w = 0; w < wave; ++w) acc += s_tmp[w];
+    for (int i = b; i < e; ++i) {
+        const int v = s[i];
+        s[i] = acc;
+        acc += v;
+    }
+    __syncthreads();
+}
+
+// Uniform grid over one contact's face candidate boxes, built by the whole
+// block in LDS: cell size = cell_scale x the largest face extent (grown
+// until the grid has at most MAXC cells); face lists per cell in s_entries
+// (order within a cell is irrelevant: the search keeps the lowest index).
+template <int B, int MAXC, int MAXE>
+__device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, double cell_scale, int* s_start,
+                                    int* s_cursor, int* s_entries, double* s_w, int* s_tmp) {
+    const int t = threadIdx.x, F = D.F, fb = cd.face_begin, nf = cd.n_faces;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY}, sz = 0.0;
+    for (int f = t; f < nf; f += B) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = fmin(lo[d], D.fgeo[(size_t)(FG_CLO + d) * F + fb + f]);
+            hi[d] = fmax(hi[d], D.fgeo[(size_t)(FG_CHI + d) * F + fb + f]);
+        }
+        sz = fmax(sz, D.fgeo[(size_t)FG_SZ * F + fb + f]);
+    }
+    FaceGrid g;
+    double ghi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        g.glo[d] = block_reduce<B>(lo[d], 1, s_w);
+        ghi[d] = block_reduce<B>(hi[d], 2, s_w);
+    }
+    sz = block_reduce<B>(sz, 2, s_w);
+    double h = fmax(sz * cell_scale, 1e-9);
+    long long cells = 1;
+    for (int iter = 0; iter < 64; ++iter) {
+        cells = 1;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double ext = ghi[d] - g.glo[d];
+            g.dim[d] = ext > 0.0 ? max((int)fmin(ceil(ext / h), 1e6), 1) : 1;
+            cells *= g.dim[d];
+        }
+        if (cells <= MAXC) break;
+        h *= 1.25;
+    }
+    g.h = h;
+    g.overflow = (nf == 0 || cells > MAXC || !(isfinite(g.glo[0]) && isfinite(ghi[0]))) ? 1 : 0;
+    const int dx = g.dim[0], dy = g.dim[1], dz = g.dim[2];
+    const int ncell = g.overflow ? 1 : dx * dy * dz;
+    for (int i = t; i < ncell + 1; i += B) s_start[i] = 0;
+    __syncthreads();
+    if (!g.overflow) {
+        for (int f = t; f < nf; f += B) {
+            double flo[3], fhi[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                flo[d] = D.fgeo[(size_t)(FG_CLO + d) * F + fb + f];
+                fhi[d] = D.fgeo[(size_t)(FG_CHI + d) * F + fb + f];
+            }
+            if (!(flo[0] <= fhi[0] && flo[1] <= fhi[1] && flo[2] <= fhi[2])) continue;
+            const int a0 = grid_cell(g, flo[0], 0), a1 = grid_cell(g, fhi[0], 0);
+            const int b0 = grid_cell(g, flo[1], 1), b1 = grid_cell(g, fhi[1], 1);
+            const int c0 = grid_cell(g, flo[2], 2), c1 = grid_cell(g, fhi[2], 2);
+            for (int kz = c0; kz <= c1; ++kz)
+                for (int ky = b0; ky <= b1; ++ky)
+                    for (int kx = a0; kx <= a1; ++kx) atomicAdd(&s_start[(kz * dy + ky) * dx + kx], 1);
+        }
+    }
+    __syncthreads();
+    block_scan<B>(s_start, ncell + 1, s_tmp);
+    if (s_start[ncell] > MAXE) g.overflow = 1;
+    for (int i = t; i < ncell; i += B) s_cursor[i] = s_start[i];
+    __syncthreads();
+    if (!g.overflow) {
+        for (int f = t; f < nf; f += B) {
+            double flo[3], fhi[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                flo[d] = D.fgeo[(size_t)(FG_CLO + d) * F + fb + f];
+                fhi[d] = D.fgeo[(size_t)(FG_CHI + d) * F + fb + f];
+            }
+            if (!(flo[0] <= fhi[0] && flo[1] <= fhi[1] && flo[2] <= fhi[2])) continue;
+            const int a0 = grid_cell(g, flo[0], 0), a1 = grid_cell(g, fhi[0], 0);
+            const int b0 = grid_cell(g, flo[1], 1), b1 = grid_cell(g, fhi[1], 1);
+            const int c0 = grid_cell(g, flo[2], 2), c1 = grid_cell(g, fhi[2], 2);
+            for (int kz = c0; kz <= c1; ++kz)
+                for (int ky = b0; ky <= b1; ++ky)
+                    for (int kx = a0; kx <= a1; ++kx) s_entries[atomicAdd(&s_cursor[(kz * dy + ky) * dx + kx], 1)] = f;
+        }
+    }
+    __syncthreads();
+    return g;
+}
+
+// Crush of group g, |u_axis(accel) - mean u_axis(front)| (src/bin/crushrs.rs:265-271).
+template <int B>
+__device__ double group_crush(const InstDev& D, int g, double* s_w) {
+    const int b = D.g_front_ptr[g], e = D.g_front_ptr[g + 1], ax = D.g_axis[g];
+    double s = 0.0;
+    for (int i = b + (int)threadIdx.x; i < e; i += B) s += D.u[3 * (size_t)D.g_front[i] + ax];
+    s = block_reduce<B>(s, 0, s_w);
+    if (D.g_accel[g] < 0 || e <= b) return 0.0;
+    return fabs(D.u[3 * (size_t)D.g_accel[g] + ax] - s / (double)(e - b));
+}
+
+// Mass-weighted mean position and velocity of group g -> out[0..6]
+// = {mass, x, y, z, vx, vy, vz}.
+template <int B>
+__device__ void group_means(const InstDev& D, int g, double* s_w, double* out) {
+    double s[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int i = D.g_nodes_ptr[g] + (int)threadIdx.x; i < D.g_nodes_ptr[g + 1]; i += B) {
+        const int n = D.g_nodes[i];
+        const double m = D.mass[n];
+        s[0] += m;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            s[1 + d] += m * (D.X[3 * (size_t)n + d] + D.u[3 * (size_t)n + d]);
+            s[4 + d] += m * D.v[3 * (size_t)n + d];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 7; ++k) out[k] = block_reduce<B>(s[k], 0, s_w);
+}
+
+// End of a step: time, step count, crush samples, separation and the stop
+// decision (the loop condition of src/solver.rs:287 plus the optional
+// retirement at separation).  c is the block's control block in LDS; only
+// thread 0 writes it, and every thread returns the same stop decision.
+template <int B>
+__device__ bool step_end(const InstDev& D, StepCtl* c, double dt, int pair_hits, bool bad, double* s_w) {
+    const long long step = c->step + 1;
+    double cr[8];
+    const int ng = D.n_groups < 8 ? D.n_groups : 8;
+    for (int g = 0; g < ng; ++g) cr[g] = group_crush<B>(D, g, s_w);
+    const int had_pair = pair_hits > 0 ? 1 : c->had_pair;
+    const int quiet = pair_hits > 0 ? 0 : c->quiet + 1;
+    bool sep = false;
+    if (c->sep_steps > 0 && D.n_groups >= 2 && had_pair && quiet >= c->sep_steps && step % c->chk == 0) {
+        double a[7], b[7];
+        group_means<B>(D, 0, s_w, a);
+        group_means<B>(D, 1, s_w, b);
+        double dot = 0.0;
+        for (int d = 0; d < 3; ++d) dot += (b[4 + d] / b[0] - a[4 + d] / a[0]) * (b[1 + d] / b[0] - a[1 + d] / a[0]);
+        sep = dot > 0.0;
+    }
+    const double tn = c->t + dt;
+    const bool done = sep || bad || !(tn < c->end_time);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int g = 0; g < ng; ++g) c->crush_max[g] = fmax(c->crush_max[g], cr[g]);
+        c->had_pair = had_pair;
+        c->quiet = quiet;
+        c->t = tn;
+        c->dt = dt;
+        c->step = step;
+        if (sep) c->separated = 1;
+        if (bad) c->error = 1;
+        if (done) c->done = 1;
+    }
+    __syncthreads();
+    return done;
+}
+
+}  // namespace crush

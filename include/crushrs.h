@@ -190,6 +190,56 @@ int32_t crush_debug_run_steps(const crush_model* model, int32_t n_steps,
  * contacts (the row length of crush_debug_run_steps' hit). */
 int32_t crush_contact_node_count(const crush_model* model, int32_t* count);
 
+
+/* ---- batched solves ----------------------------------------------------
+ * Many independent models solved together on one GPU: the counterpart of
+ * the reference's calibration sweeps, which run solver::run for each model
+ * in a rayon par_iter (src/vehicle.rs:1227-1233, src/headon.rs:179-182).
+ * Each instance is solved exactly as crush_run would (same kernels' per-item
+ * arithmetic); results do not depend on the other instances in the batch. */
+
+typedef struct crush_batch crush_batch;
+
+typedef struct crush_batch_options {
+    int32_t threads_per_instance; /* workgroup size per instance: 256, 512 or 1024 (0: 256) */
+    int32_t steps_per_launch;     /* steps per instance per launch (0: to completion) */
+    int32_t launches_per_graph;   /* launches captured in one hipGraph (>= 1) */
+    int32_t host_threads;         /* host threads preparing models (0: all cores, max 16) */
+    double contact_cell_scale;    /* contact grid cell / largest face extent (0: 1.0) */
+} crush_batch_options;
+
+typedef struct crush_batch_stats {
+    double prepare_ms;            /* host preparation of all models           */
+    double upload_ms;             /* host->device copy of the instance images */
+    double device_ms;             /* device time of crush_batch_run           */
+    double wall_ms;               /* host wall time of crush_batch_run        */
+    int64_t instances;
+    int64_t element_steps;        /* sum over instances of hexes x steps      */
+    int64_t launches;             /* kernel launches of the batched step      */
+    int64_t graph_replays;
+    double pool_bytes;            /* device memory of the instance images     */
+} crush_batch_stats;
+
+/* Fill opt with the defaults. */
+void crush_batch_defaults(crush_batch_options* opt);
+
+/* Prepare n models (host threads) and upload them to the device. */
+int32_t crush_batch_create(const crush_model* const* models, int32_t n, const crush_batch_options* opt,
+                           crush_batch** out);
+
+/* Solve every instance to its end (end_time or separation). */
+int32_t crush_batch_run(crush_batch* b, crush_batch_stats* stats);
+
+/* Restore every instance to its initial state (so the batch can be run
+ * again, e.g. for timing). */
+int32_t crush_batch_reset(crush_batch* b);
+
+/* Results of instance i (fields as crush_run; the displacement and
+ * velocity arrays are copied only when the pointers are non-NULL). */
+int32_t crush_batch_result(crush_batch* b, int32_t i, crush_results* out);
+
+void crush_batch_free(crush_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

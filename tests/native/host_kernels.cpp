@@ -28,9 +28,7 @@ extern "C" void hk_hencky(const float* hd, float youngs, float nu, float* tau) {
 
 extern "C" void hk_honeycomb(const float* hd, float youngs, const float* knots, float trans, float* st, float* tau) {
     MatParams mp = make(youngs, 0.0f, knots, trans);
-    float F[9];
-    for (int k = 0; k < 9; ++k) F[k] = hd[k] + ((k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f);
-    honeycomb_update(mp, hd, F, st, tau);
+    honeycomb_update(mp, hd, st, tau);
 }
 
 extern "C" void hk_eigen(const float* a, float* vals, float* vecs) { sym_eigen3f(a, vals, vecs); }
