@@ -1,48 +1,232 @@
 """Benchmark of the crash-solver core on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--mode plain]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sweep|crash]
+                    [--instances N] [--no-cpu] [--no-plain]
 
-One bench step is one complete solve of the workload.  Prints one JSON line.
+Workloads
+  sweep  (default) the batched sweep: N (4096) independent head-on/offset
+         pairs of the four tuned vehicle TOMLs, speeds U(10, 40) mph each,
+         lateral offset U(0, 0.6) m, element size 0.3 or 0.2 m, rigid road,
+         0.15 s each, solved together by the batched persistent kernel.
+         One bench step is one batched solve of every instance to its end.
+  crash  plain single solve: `crush crash 30 30` (Silverado vs Neon presets,
+         1290 hexes, no road) with the one-model path.  One bench step is
+         one solve.
+
+Prints one JSON line.  `value` is solves/s of the workload with every model
+already resident in device memory; preparation and upload are reported
+beside it, not inside it.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
+import platform
 import sys
 import time
+from concurrent.futures import ThreadPoolExecutor
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
-sys.path.insert(0, str(ROOT / "tests"))
 
 import numpy as np  # noqa: E402
 
 import crushrs_amd as ca  # noqa: E402
+from crushrs_amd import workloads  # noqa: E402
 
 METRIC = "batched crash solves/s + p50/p99 latency vs plain GPU; Gelem-steps/s, %HBM"
 HBM_PEAK_GBS = 8000.0
 
+# Algorithmic bytes of one explicit step (f32 element data, f64 nodal data):
+# force pass 776 B per honeycomb element (connectivity 32, gradients/volume/
+# hourglass constants 540, state 100 read + 100 written, part/erosion 4),
+# 576 B per elastic element (no state), 36 B per node (gather X and u, write
+# f_int); central-difference update 76 B per node; the stable-step pass every
+# adaptive_check_steps steps 36 B per element and 24 B per node.  Contact is
+# not counted.
+B_HC, B_EL, B_NODE_FORCE, B_NODE_UPDATE = 776, 576, 36, 76
+B_DT_ELEM, B_DT_NODE = 36, 24
 
-def cpu_baseline_plain(model, budget_s: float = 15.0):
-    """The f64 oracle (single core) on the same model, bounded sample."""
+
+def step_bytes(n_hc: int, n_el: int, n_nodes: int, steps: int, check_every: int = 5) -> float:
+    per = B_HC * n_hc + B_EL * n_el + (B_NODE_FORCE + B_NODE_UPDATE) * n_nodes
+    dt = (B_DT_ELEM * (n_hc + n_el) + B_DT_NODE * n_nodes) * -(-steps // check_every)
+    return float(per) * steps + dt
+
+
+def cpu_info() -> str:
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(models, elem_steps_per_solve: float, budget_s: float = 20.0, sample_steps: int = 100):
+    """The f64 scalar restatement (oracle) on the host cores: each thread
+    runs `sample_steps` explicit steps of one sampled model at a time; the
+    step rate, less the set-up time of a zero-step call, is converted to
+    solves/s of this workload through its mean element-steps per solve."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import ctypes as C
     import oracle_ffi as of
+    lib = of.load()
+    cores = max(1, min(os.cpu_count() or 1, 16))
+    rng = np.random.default_rng(1)
+    order = rng.permutation(len(models))
+    t_end = time.perf_counter() + budget_s
+
+    def worker(w):
+        busy, es, k = 0.0, 0.0, 0
+        done = C.c_int32()
+        for j in order[w::cores]:
+            if time.perf_counter() > t_end and k > 0:
+                break
+            m = models[j]
+            t0 = time.perf_counter()
+            lib.oracle_run_steps(m.ptr, 0, None, None, None, None, None, C.byref(done))
+            t1 = time.perf_counter()
+            lib.oracle_run_steps(m.ptr, sample_steps, None, None, None, None, None, C.byref(done))
+            t2 = time.perf_counter()
+            busy += max((t2 - t1) - (t1 - t0), 1e-9)
+            es += m.n_hexes * done.value
+            k += 1
+        return es / busy, k
+
+    with ThreadPoolExecutor(max_workers=cores) as ex:
+        res = list(ex.map(worker, range(cores)))
+    rate = sum(r for r, _ in res)
+    n = sum(k for _, k in res)
+    return {
+        "value": rate / elem_steps_per_solve,
+        "unit": "solves/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"f64 scalar restatement of the reference's solver (kernel.rs path), {n} sampled models x "
+                   f"{sample_steps} steps on {cores} threads of {cpu_info()}; {rate / 1e6:.2f} M element-steps/s "
+                   f"converted with this workload's mean {elem_steps_per_solve:.3g} element-steps per solve"),
+        "gelem_steps_per_s": rate / 1e9,
+    }
+
+
+def plain_gpu(models, k: int = 8):
+    """The one-model GPU path (crush_run, one solve at a time) on k sampled
+    models of the workload."""
+    rng = np.random.default_rng(2)
+    idx = rng.choice(len(models), size=min(k, len(models)), replace=False)
+    ca.run(models[int(idx[0])])  # load kernels
+    t, es = [], 0.0
+    for i in idx:
+        s = time.perf_counter()
+        r = ca.run(models[int(i)])
+        t.append(time.perf_counter() - s)
+        es += models[int(i)].n_hexes * r.steps
+    tot = sum(t)
+    return {"value": len(t) / tot, "unit": "solves/s", "sample": f"{len(t)} sampled models solved one at a time",
+            "gelem_steps_per_s": es / tot / 1e9, "ms_per_solve_p50": float(np.median(t) * 1e3)}
+
+
+def bench_sweep(args):
     t0 = time.perf_counter()
-    n = 0
-    steps = 0
-    while time.perf_counter() - t0 < budget_s:
-        r = of.run(model)
-        steps = r.steps
-        n += 1
-        if n >= 3:
-            break
-    dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"{n} full solves of the same model with the f64 scalar oracle ({steps} steps each)"}
+    cases = workloads.headon_sweep(args.instances, seed=args.seed)
+    models = workloads.build_models(cases)
+    t_build = time.perf_counter() - t0
+    n_hc = np.array([workloads.honeycomb_count(m) for m in models])
+    E = np.array([m.n_hexes for m in models])
+    N = np.array([m.n_nodes for m in models])
+    opt = ca.batch_options(threads_per_instance=args.threads_per_instance, force_tile=args.force_tile,
+                           steps_per_graph=args.steps_per_graph, autotune=int(args.autotune))
+    batch = ca.Batch(models, opt)
+    for _ in range(args.warmup):
+        batch.run()
+        batch.reset()
+    dev_ms, wall, lat50, lat99, launches = [], [], [], [], 0
+    for k in range(args.steps):
+        if k:
+            batch.reset()
+        st = batch.run()
+        dev_ms.append(st.device_ms)
+        wall.append(st.wall_ms)
+        lat50.append(st.latency_p50_ms)
+        lat99.append(st.latency_p99_ms)
+        launches = st.launches
+    steps = np.array([batch.result(i, fields=False).steps for i in range(len(models))])
+    n = len(models)
+    es = float((E * steps).sum())
+    t_run = sum(wall) / 1e3
+    value = n * args.steps / t_run
+    total_bytes = sum(step_bytes(int(n_hc[i]), int(E[i] - n_hc[i]), int(N[i]), int(steps[i])) for i in range(n))
+    # roofline of the dominant kernel, the honeycomb element-force tiles:
+    # a second solve of the same sweep with HIP events recorded around every
+    # launch (on the stream the kernels run on) gives the summed duration
+    batch.time_kernels(True)
+    batch.reset()
+    tk = batch.run()
+    batch.time_kernels(False)
+    k_launches = max(1, tk.launches)
+    avg_ms = tk.k_force_hc_ms / k_launches
+    bytes_per_launch = B_HC * tk.hc_element_steps / k_launches
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    el_steps = tk.element_steps - tk.hc_element_steps
+    kernels = {
+        "k_inst_ms": tk.k_inst_ms, "k_node_ms": tk.k_node_ms, "k_force_hc_ms": tk.k_force_hc_ms,
+        "k_force_el_ms": tk.k_force_el_ms,
+        "launches_each": tk.launches,
+        "force_el_gbs": B_EL * el_steps / max(tk.k_force_el_ms, 1e-9) / 1e6,
+        "whole_step_algorithmic_gbs": total_bytes / (tk.device_ms * 1e-3) / 1e9,
+    }
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_run / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded head-on/offset pairs of the reference's four vehicle TOMLs, generated meshes",
+        "config": {
+            "workload": (f"batched sweep: {n} head-on/offset pairs (neon/silverado/expedition/navigator TOMLs), "
+                         "speeds U(10,40) mph, offset U(0,0.6) m, h in {0.3,0.2} m, rigid road, 0.15 s"),
+            "instances": n, "seed": args.seed,
+            "elements": {"min": int(E.min()), "max": int(E.max()), "mean": float(E.mean())},
+            "solver_steps": {"min": int(steps.min()), "max": int(steps.max()), "mean": float(steps.mean())},
+            "threads_per_instance": batch.options.threads_per_instance, "force_tile": batch.options.force_tile,
+            "contact_cell_scale": batch.options.contact_cell_scale, "steps_per_graph": batch.options.steps_per_graph,
+        },
+        "gelem_steps_per_s": es * args.steps / t_run / 1e9,
+        "latency_ms": {"p50": float(np.mean(lat50)), "p99": float(np.mean(lat99)),
+                       "definition": "batch start to instance retirement"},
+        "device_ms": float(np.mean(dev_ms)),
+        "host": {"build_models_s": t_build, "prepare_ms": st.prepare_ms, "upload_ms": st.upload_ms,
+                 "pool_gib": st.pool_bytes / 2**30},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_force_tiles<honeycomb>", "launch_ms": avg_ms,
+                     "algorithmic_bytes_per_launch": bytes_per_launch, "bytes_per_element_step": B_HC},
+        "kernels": kernels,
+    }
+    if not args.no_plain:
+        pg = plain_gpu(models)
+        out["plain_gpu"] = pg
+        out["speedup_vs_plain"] = value / pg["value"]
+    if not args.no_cpu:
+        cb = cpu_baseline(models, es / n)
+        out["cpu_baseline"] = cb
+        out["speedup_vs_cpu"] = value / cb["value"]
+    else:
+        out["cpu_baseline"] = None
+    return out
 
 
-def bench_plain(args):
+def bench_crash(args):
     m = ca.Model.crash(30.0, 30.0)
     for _ in range(args.warmup):
         ca.run(m)
@@ -73,7 +257,7 @@ def bench_plain(args):
         "gelem_steps_per_s": elem_steps * args.steps / wall / 1e9,
         "latency_ms": {"p50": float(np.percentile(times, 50) * 1e3), "p99": float(np.percentile(times, 99) * 1e3)},
         "roofline": None,
-        "cpu_baseline": cpu_baseline_plain(m) if not args.no_cpu else None,
+        "cpu_baseline": None if args.no_cpu else cpu_baseline([m], float(elem_steps), budget_s=10.0),
     }
     return out
 
@@ -81,20 +265,34 @@ def bench_plain(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--mode", default="plain", choices=["plain"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="sweep", choices=["sweep", "crash"])
+    ap.add_argument("--instances", type=int, default=4096)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--threads-per-instance", type=int, default=256)
+    ap.add_argument("--force-tile", type=int, default=256)
+    ap.add_argument("--steps-per-graph", type=int, default=32)
+    ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; independent instances would shard across GPUs "
-                                   "with no communication (see DESIGN.md), but only --gpus 1 is measured"}))
+                                   "with no communication, but only --gpus 1 is built and measured"}))
         sys.exit(2)
     if ca.device_count() < 1:
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(1)
-    out = bench_plain(args)
-    print(json.dumps(out))
+    if args.workload == "sweep":
+        args.steps = 1 if args.steps is None else args.steps
+        args.warmup = 1 if args.warmup is None else args.warmup
+        out = bench_sweep(args)
+    else:
+        args.steps = 20 if args.steps is None else args.steps
+        args.warmup = 2 if args.warmup is None else args.warmup
+        out = bench_crash(args)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -307,6 +307,15 @@ class Batch:
     def reset(self) -> None:
         check(self._lib.crush_batch_reset(self._h))
 
+    def time_kernels(self, on: bool = True) -> None:
+        check(self._lib.crush_batch_time_kernels(self._h, int(on)))
+
+    @property
+    def options(self) -> _abi.BatchOptions:
+        o = _abi.BatchOptions()
+        check(self._lib.crush_batch_options_get(self._h, C.byref(o)))
+        return o
+
     def result(self, i: int, fields: bool = True) -> Results:
         r, u, v, m, g = alloc_results(self._n_nodes[i], self._n_groups[i])
         if not fields:

@@ -175,10 +175,12 @@ class HeadOnOptions(C.Structure):
 class BatchOptions(C.Structure):
     _fields_ = [
         ("threads_per_instance", C.c_int32),
-        ("steps_per_launch", C.c_int32),
-        ("launches_per_graph", C.c_int32),
+        ("force_tile", C.c_int32),
+        ("steps_per_graph", C.c_int32),
         ("host_threads", C.c_int32),
         ("contact_cell_scale", C.c_double),
+        ("autotune", C.c_int32),
+        ("reserved0", C.c_int32),
     ]
 
 
@@ -193,6 +195,13 @@ class BatchStats(C.Structure):
         ("launches", C.c_int64),
         ("graph_replays", C.c_int64),
         ("pool_bytes", C.c_double),
+        ("latency_p50_ms", C.c_double),
+        ("latency_p99_ms", C.c_double),
+        ("hc_element_steps", C.c_int64),
+        ("k_inst_ms", C.c_double),
+        ("k_node_ms", C.c_double),
+        ("k_force_hc_ms", C.c_double),
+        ("k_force_el_ms", C.c_double),
     ]
 
 
@@ -211,6 +220,8 @@ SIGNATURES = {
                                        C.POINTER(C.c_void_p)]),
     "crush_batch_run": (C.c_int32, [C.c_void_p, C.POINTER(BatchStats)]),
     "crush_batch_reset": (C.c_int32, [C.c_void_p]),
+    "crush_batch_options_get": (C.c_int32, [C.c_void_p, C.POINTER(BatchOptions)]),
+    "crush_batch_time_kernels": (C.c_int32, [C.c_void_p, C.c_int32]),
     "crush_batch_result": (C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(Results)]),
     "crush_batch_free": (None, [C.c_void_p]),
     "crush_headon_defaults": (None, [C.POINTER(HeadOnOptions)]),

@@ -1,15 +1,22 @@
 // Batched solves: thousands of independent models on one GPU.
 //
 // Each instance is one contiguous image in a device pool (its arrays plus a
-// control block) described by an InstDev pointer table.  A persistent grid
-// of workgroups pulls instance indices from a device work queue; the
-// workgroup that takes an instance runs up to S explicit steps of it, every
-// phase of a step (stable step, contact, half step, element forces, gather,
-// end of step) as block-strided loops separated by barriers.  No phase
-// depends on other instances, so an instance's results are bitwise the same
-// alone or in any mix.  The launch sequence (queue reset + persistent
-// kernel, R times) is captured in one hipGraph and replayed until every
-// instance has retired.
+// control block) described by an InstDev pointer table.  One explicit step
+// of every active instance is five launches:
+//   k_inst          one workgroup per instance: end of the previous step,
+//                   stable step (at check steps) and contact;
+//   k_half_tiles    first half step, one workgroup per tile of 256 nodes of
+//                   one instance, over all instances at once;
+//   k_force_tiles   element forces (and the next check step's stable-step
+//                   partials), one workgroup per tile of element slots of
+//                   one instance, honeycomb and elastic tiles in separate
+//                   launches so each fills the GPU at its own register
+//                   budget;
+//   k_gather_tiles  node-centric gather and second half step, node tiles.
+// No arithmetic of an instance depends on the others, so its results are
+// bitwise the same alone or in any mix.  A fixed number of steps is
+// captured in one hipGraph and replayed until every instance has retired;
+// retired instances leave the active lists between replays.
 #include "batch.hpp"
 
 #include <hip/hip_runtime.h>
@@ -230,45 +237,15 @@ static InstDev make_inst(const Prepared& P, const ImageOffsets& o, uintptr_t bas
     return I;
 }
 
-// ------------------------------------------------------------- kernel
+// ------------------------------------------------------------- kernels
 
-// Phases of one step for one instance, block-strided, each ending with a
-// barrier.
+// Contact of one instance (src/contact.rs:92-160 for each contact in
+// order): face geometry, per-contact LDS grid and node search, face
+// reactions and the per-node sums.  Returns the number of penetrating
+// nodes of pair contacts seen by this thread.
 template <int B>
-__device__ __forceinline__ void ph_dt(const InstDev* Dp, StepCtl* c, double* s_w) {
-    const InstDev& D = *Dp;
-    const int t = threadIdx.x;
-    double mn = INFINITY, sm = 0.0;
-    const double floor_ = c->dt_floor;
-    for (int s = t; s < D.E; s += B) {
-        double dte, ex;
-        dt_element(D, s, floor_, &dte, &ex);
-        mn = fmin(mn, dte);
-        sm += ex;
-    }
-    mn = block_reduce<B>(mn, 1, s_w);
-    sm = block_reduce<B>(sm, 0, s_w);
-    if (isfinite(mn)) {
-        double dt_use = mn;
-        const bool scale = floor_ > 0.0 && mn < floor_;
-        if (scale) {
-            dt_use = floor_;
-            for (int n = t; n < D.N; n += B) mass_node(D, n);
-        }
-        const double cfl = c->dt_scale * fmin(dt_use, c->contact_dt);
-        __syncthreads();
-        if (t == 0) {
-            if (scale) c->added_total_max = fmax(c->added_total_max, sm);
-            c->dt_cfl = cfl;
-        }
-        __syncthreads();
-    }
-}
-
-template <int B>
-__device__ __forceinline__ double ph_contact(const InstDev* Dp, double dt, double cell_scale, int* s_start,
-                                                        int* s_cursor, int* s_entries, double* s_w, int* s_tmp) {
-    const InstDev& D = *Dp;
+__device__ __forceinline__ double ph_contact(const InstDev& D, double dt, double cell_scale, int* s_start,
+                                             int* s_cursor, int* s_entries, double* s_w, int* s_tmp) {
     const int t = threadIdx.x;
     double pair_hits = 0.0;
     for (int f = t; f < D.F; f += B) contact_face(D, f);
@@ -288,109 +265,195 @@ __device__ __forceinline__ double ph_contact(const InstDev* Dp, double dt, doubl
     return pair_hits;
 }
 
+// Outputs of a retired instance: per-group delta-v and end crush.
 template <int B>
-__device__ __forceinline__ void ph_half(const InstDev* Dp, double dt) {
-    const InstDev& D = *Dp;
-    for (int n = threadIdx.x; n < D.N; n += B) half_node(D, n, dt);
-    __syncthreads();
-}
-
-template <int B>
-__device__ __forceinline__ double ph_force(const InstDev* Dp) {
-    const InstDev& D = *Dp;
-    double eroded = 0.0;
-    for (int s = threadIdx.x; s < D.n_hc; s += B) eroded += force_element<CRUSH_MAT_HONEYCOMB>(D, s);
-    for (int s = D.n_hc + threadIdx.x; s < D.E; s += B) eroded += force_element<CRUSH_MAT_ELASTIC>(D, s);
-    __syncthreads();
-    return eroded;
-}
-
-template <int B>
-__device__ __forceinline__ double ph_gather(const InstDev* Dp, double dt, double damp) {
-    const InstDev& D = *Dp;
-    double bad = 0.0;
-    for (int n = threadIdx.x; n < D.N; n += B)
-        if (!gather_node(D, n, dt, damp)) bad = 1.0;
-    __syncthreads();
-    return bad;
-}
-
-// Up to S steps of one instance by the whole block (the loop body of
-// src/solver.rs:287-352).  Returns true when the instance has retired.
-template <int B>
-__device__ bool run_instance(const InstDev* Dp, int S, double cell_scale, int* s_start, int* s_cursor, int* s_entries,
-                             double* s_w, int* s_tmp, StepCtl* c) {
-    const InstDev& D = *Dp;
+__device__ void finish_instance(const InstDev& D, StepCtl* c, double* s_w) {
     const int t = threadIdx.x;
-    if (t == 0) *c = *D.ctl;
-    __syncthreads();
-    if (c->done) return false;
-    for (int k = 0; k < S && !c->done; ++k) {
-        if (c->adaptive && c->step % c->chk == 0) ph_dt<B>(Dp, c, s_w);
-        const double dt = step_dt(*c);
-        double pair_hits = 0.0;
-        if (D.F > 0) pair_hits = ph_contact<B>(Dp, dt, cell_scale, s_start, s_cursor, s_entries, s_w, s_tmp);
-        ph_half<B>(Dp, dt);
-        double eroded = ph_force<B>(Dp);
-        double bad = ph_gather<B>(Dp, dt, c->damping);
-        pair_hits = block_reduce<B>(pair_hits, 0, s_w);
-        eroded = block_reduce<B>(eroded, 0, s_w);
-        bad = block_reduce<B>(bad, 2, s_w);
-        if (t == 0) c->eroded_total += (unsigned long long)eroded;
-        step_end<B>(D, c, dt, (int)pair_hits, bad > 0.0, s_w);
-    }
-    const bool done = c->done != 0;
-    if (done) {
-        for (int g = 0; g < D.n_groups; ++g) {
-            double m[7];
-            group_means<B>(D, g, s_w, m);
-            if (t == 0)
-                for (int d = 0; d < 3; ++d) D.g_dv[3 * g + d] = m[0] > 0.0 ? m[4 + d] / m[0] - D.g_v0[3 * g + d] : 0.0;
-            if (g < 8) {
-                const double cr = group_crush<B>(D, g, s_w);
-                if (t == 0) {
-                    c->crush_end[g] = cr;
-                    c->crush_max[g] = fmax(c->crush_max[g], cr);
-                }
+    if (t == 0) c->t_retire = wall_clock64();
+    for (int g = 0; g < D.n_groups; ++g) {
+        double m[7];
+        group_means<B>(D, g, s_w, m);
+        if (t == 0)
+            for (int d = 0; d < 3; ++d) D.g_dv[3 * g + d] = m[0] > 0.0 ? m[4 + d] / m[0] - D.g_v0[3 * g + d] : 0.0;
+        if (g < 8) {
+            const double cr = group_crush<B>(D, g, s_w);
+            if (t == 0) {
+                c->crush_end[g] = cr;
+                c->crush_max[g] = fmax(c->crush_max[g], cr);
             }
         }
     }
-    __syncthreads();
-    if (t == 0) *D.ctl = *c;
-    __syncthreads();
-    return done;
 }
 
+// Slot of a force tile among the stable-step partials of its instance:
+// honeycomb tiles first, then elastic ones.
+template <int KIND, int T>
+__device__ __forceinline__ int dt_slot(const InstDev& D, int first) {
+    return KIND == CRUSH_MAT_HONEYCOMB ? first / T : (D.n_hc + T - 1) / T + (first - D.n_hc) / T;
+}
+
+// Stable-step partial of one tile (src/kernel.rs:207-258): min of the
+// element steps and sum of the scaling masses, in a fixed order.  Every
+// thread of the block calls it.
+template <int T>
+__device__ __forceinline__ void dt_partial(const InstDev& D, double dt_floor, int s, int end, int slot, double* s_w) {
+    double mn = INFINITY, ex = 0.0;
+    if (s < end) dt_element(D, s, dt_floor, &mn, &ex);
+    mn = block_reduce<T>(mn, 1, s_w);
+    ex = block_reduce<T>(ex, 0, s_w);
+    if (threadIdx.x == 0) {
+        D.dtp_min[slot] = mn;
+        D.dtp_sum[slot] = ex;
+    }
+}
+
+// Per-instance part of a step (src/solver.rs:287-352), one workgroup per
+// active instance.  It closes the open step (end-of-step bookkeeping after
+// the gather) and opens the next: the stable step from the tile partials
+// at check steps, then contact.  `pending` marks an open step whose half
+// step, element forces and gather are due (k_half_tiles, k_force_tiles,
+// k_gather_tiles).
 template <int B>
-__global__ __launch_bounds__(B) void k_batch(const InstDev* __restrict__ table, const int32_t* __restrict__ order, int n,
-                                             int S, double cell_scale, int* __restrict__ counter,
-                                             int* __restrict__ n_done) {
+__global__ __launch_bounds__(B) void k_inst(const InstDev* __restrict__ table, const int32_t* __restrict__ list,
+                                            const int32_t* __restrict__ n_list, double cell_scale, int force_tile,
+                                            uint8_t* __restrict__ done_flag, int* __restrict__ n_done) {
     __shared__ int s_start[BATCH_GRID_CELLS + 1];
     __shared__ int s_cursor[BATCH_GRID_CELLS];
     __shared__ int s_entries[BATCH_GRID_ENTRIES];
     __shared__ double s_w[B / 64];
     __shared__ int s_tmp[B / 64];
-    __shared__ int s_next;
     __shared__ StepCtl s_ctl;
-    __shared__ InstDev s_inst;
-    for (;;) {
-        if (threadIdx.x == 0) s_next = atomicAdd(counter, 1);
+    if ((int)blockIdx.x >= *n_list) return;
+    const int inst = list[blockIdx.x];
+    const InstDev& D = table[inst];
+    StepCtl* c = &s_ctl;
+    const int t = threadIdx.x;
+    if (t == 0) *c = *D.ctl;
+    __syncthreads();
+    if (c->done) return;
+    if (c->pending) {
+        const double dt = step_dt(*c);
+        const int hits = c->pair_hits;
+        const bool bad = c->bad_step != 0;
         __syncthreads();
-        const int i = s_next;
-        if (i >= n) return;
-        // the instance's pointer table lives in LDS and is read where used
-        constexpr int W = (int)(sizeof(InstDev) / sizeof(int));
-        const int idx = order[i];
-        for (int w = threadIdx.x; w < W; w += B)
-            reinterpret_cast<int*>(&s_inst)[w] = reinterpret_cast<const int*>(&table[idx])[w];
+        if (t == 0) {
+            c->eroded_total += (unsigned long long)c->eroded_step;
+            c->eroded_step = 0;
+            c->bad_step = 0;
+            c->pending = 0;
+        }
         __syncthreads();
-        if (run_instance<B>(&s_inst, S, cell_scale, s_start, s_cursor, s_entries, s_w, s_tmp, &s_ctl) && threadIdx.x == 0)
-            atomicAdd(n_done, 1);
-        __syncthreads();
+        if (step_end<B>(D, c, dt, hits, bad, s_w)) {
+            finish_instance<B>(D, c, s_w);
+            if (t == 0) {
+                done_flag[inst] = 1;
+                atomicAdd(n_done, 1);
+            }
+        }
     }
+    if (!c->done) {
+        if (t == 0) {
+            c->scale_now = 0;
+            if (c->adaptive && c->step % c->chk == 0) {
+                // stable step and selective mass scaling (src/solver.rs:288-299)
+                const int T = force_tile;
+                const int slots = (D.n_hc + T - 1) / T + (D.E - D.n_hc + T - 1) / T;
+                double mn = INFINITY, sm = 0.0;
+                for (int j = 0; j < slots; ++j) {
+                    mn = fmin(mn, D.dtp_min[j]);
+                    sm += D.dtp_sum[j];
+                }
+                if (isfinite(mn)) {
+                    const bool scale = c->dt_floor > 0.0 && mn < c->dt_floor;
+                    if (scale) {
+                        c->added_total_max = fmax(c->added_total_max, sm);
+                        c->scale_now = 1;
+                    }
+                    c->dt_cfl = c->dt_scale * fmin(scale ? c->dt_floor : mn, c->contact_dt);
+                }
+            }
+        }
+        __syncthreads();
+        const double dt = step_dt(*c);
+        double hits = 0.0;
+        if (D.F > 0) hits = ph_contact<B>(D, dt, cell_scale, s_start, s_cursor, s_entries, s_w, s_tmp);
+        hits = block_reduce<B>(hits, 0, s_w);
+        if (t == 0) {
+            c->pair_hits = (int)hits;
+            c->pending = 1;
+        }
+    }
+    __syncthreads();
+    if (t == 0) *D.ctl = *c;
 }
 
-__global__ void k_queue_reset(int* counter) { *counter = 0; }
+// First half step of every node of the open steps (src/solver.rs:313-323),
+// after the scaling mass of the node at steps that scale (src/kernel.rs:
+// 243-258): one workgroup per tile of T nodes of one instance.
+template <int T>
+__global__ __launch_bounds__(T) void k_half_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
+                                                  const int32_t* __restrict__ n_tiles) {
+    if ((int)blockIdx.x >= *n_tiles) return;
+    const int2 tile = tiles[blockIdx.x];
+    const InstDev& D = table[tile.x];
+    const StepCtl& c = *D.ctl;
+    if (!c.pending) return;
+    const int n = tile.y + (int)threadIdx.x;
+    if (n >= D.N) return;
+    if (c.scale_now) mass_node(D, n);
+    half_node(D, n, step_dt(c));
+}
+
+// Element forces of the open steps: one workgroup per tile of T element
+// slots of one instance and one material kind.  When the next step is a
+// check step, the tile's stable-step partial is formed at the same
+// configuration (the positions and erosion the next step starts from).
+template <int KIND, int T>
+__global__ __launch_bounds__(T) void k_force_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
+                                                   const int32_t* __restrict__ n_tiles) {
+    __shared__ double s_w[T / 64];
+    if ((int)blockIdx.x >= *n_tiles) return;
+    const int2 tile = tiles[blockIdx.x];
+    const InstDev& D = table[tile.x];
+    StepCtl& c = *D.ctl;
+    if (!c.pending) return;
+    const int end = KIND == CRUSH_MAT_HONEYCOMB ? D.n_hc : D.E;
+    const int s = tile.y + (int)threadIdx.x;
+    int er = 0;
+    if (s < end) er = force_element<KIND>(D, s);
+    const unsigned long long m = __ballot(er != 0);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&c.eroded_step, (int)__popcll(m));
+    if (c.adaptive && (c.step + 1) % c.chk == 0) dt_partial<T>(D, c.dt_floor, s, end, dt_slot<KIND, T>(D, tile.y), s_w);
+}
+
+// Stable-step partials of instances that have not taken a step yet.
+template <int KIND, int T>
+__global__ __launch_bounds__(T) void k_dt_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
+                                                const int32_t* __restrict__ n_tiles) {
+    __shared__ double s_w[T / 64];
+    if ((int)blockIdx.x >= *n_tiles) return;
+    const int2 tile = tiles[blockIdx.x];
+    const InstDev& D = table[tile.x];
+    const StepCtl& c = *D.ctl;
+    if (c.done || c.pending || c.step != 0 || !c.adaptive) return;
+    const int end = KIND == CRUSH_MAT_HONEYCOMB ? D.n_hc : D.E;
+    dt_partial<T>(D, c.dt_floor, tile.y + (int)threadIdx.x, end, dt_slot<KIND, T>(D, tile.y), s_w);
+}
+
+// Gather of the element forces and second half step of the open steps
+// (src/solver.rs:327-342).
+template <int T>
+__global__ __launch_bounds__(T) void k_gather_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
+                                                    const int32_t* __restrict__ n_tiles) {
+    if ((int)blockIdx.x >= *n_tiles) return;
+    const int2 tile = tiles[blockIdx.x];
+    const InstDev& D = table[tile.x];
+    StepCtl& c = *D.ctl;
+    if (!c.pending) return;
+    const int n = tile.y + (int)threadIdx.x;
+    if (n >= D.N) return;
+    if (!gather_node(D, n, step_dt(c), c.damping)) atomicOr(&c.bad_step, 1);
+}
 
 // Compact per-instance summary for the host.
 __global__ void k_collect(const InstDev* __restrict__ table, int n, BatchSummary* __restrict__ out) {
@@ -406,12 +469,15 @@ __global__ void k_collect(const InstDev* __restrict__ table, int n, BatchSummary
     r.done = c.done;
     r.error = c.error;
     r.separated = c.separated;
+    r.t_retire = c.t_retire;
     for (int g = 0; g < 8; ++g) {
         r.crush_max[g] = g < D.n_groups ? c.crush_max[g] : 0.0;
         r.crush_end[g] = g < D.n_groups ? c.crush_end[g] : 0.0;
         for (int d = 0; d < 3; ++d) r.dv[g][d] = g < D.n_groups ? D.g_dv[3 * g + d] : 0.0;
     }
 }
+
+__global__ void k_stamp(unsigned long long* p) { *p = wall_clock64(); }
 
 // ------------------------------------------------------------- host
 
@@ -420,55 +486,243 @@ struct BatchImpl {
     std::vector<Prepared> prep;
     std::vector<ImageOffsets> offs;
     std::vector<size_t> base_off;
-    std::vector<int32_t> order;
     DevBuf<uint8_t> pool;
     DevBuf<InstDev> table;
-    DevBuf<int32_t> d_order, counter, n_done;
+    DevBuf<double> dtp;  // stable-step partials of every instance (min, then sum)
+    // active lists (instances; honeycomb, elastic and node tiles) and their
+    // counts, device ints read by the kernels
+    DevBuf<int32_t> d_list, d_counts, n_done;
+    DevBuf<int2> d_tiles_hc, d_tiles_el, d_tiles_node;
+    DevBuf<uint8_t> done_flag;
+    std::vector<uint8_t> host_done;
+    int listed = 0;  // instances in the current active list
+    int n_hc = 0, n_el = 0, n_node = 0;  // tiles in the current lists
+    int cap_inst = 0, cap_hc = 0, cap_el = 0, cap_node = 0;  // grids of the captured graph
+    bool fresh = true;  // no step taken since creation or reset
     DevBuf<BatchSummary> summary;
     std::vector<BatchSummary> host_summary;
     bool summary_valid = false;
+    DevBuf<unsigned long long> t_start;
+    unsigned long long host_t_start = 0;
+    double clock_khz = 1.0;
     hipStream_t stream = nullptr;
     hipGraphExec_t graph = nullptr;
-    int grid = 0;
     double prepare_ms = 0, upload_ms = 0;
     long long launches = 0, replays = 0;
+    // kernel timing: with it on, each replay launches its steps directly,
+    // with events ev[6r..6r+5] before each of the five kernels of step r
+    // and after the last
+    bool time_kernels = false;
+    std::vector<hipEvent_t> ev;
+    double t_inst = 0, t_half = 0, t_hc = 0, t_el = 0, t_gather = 0;
 
     ~BatchImpl() {
         if (graph) (void)hipGraphExecDestroy(graph);
+        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
-static int block_of(const crush_batch_options& o) {
-    int b = o.threads_per_instance;
-    return (b == 512 || b == 1024) ? b : 256;
+constexpr int NODE_TILE = 256;
+constexpr int EV_PER_STEP = 6;
+
+static int inst_block(const crush_batch_options& o) {
+    const int b = o.threads_per_instance;
+    return (b == 64 || b == 128 || b == 512) ? b : 256;
+}
+
+static int force_tile(const crush_batch_options& o) {
+    const int t = o.force_tile;
+    return (t == 64 || t == 128) ? t : 256;
+}
+
+static int steps_per_graph(const crush_batch_options& o) { return o.steps_per_graph > 0 ? o.steps_per_graph : 32; }
+
+// Stable-step partial slots of one instance (enough for the smallest tile).
+static size_t dtp_slots(const Prepared& P) { return (size_t)(P.n_hc + 63) / 64 + (P.n_elems - P.n_hc + 63) / 64 + 2; }
+
+// Rebuild the active lists from the instances not yet retired.
+static void rebuild_lists(BatchImpl& bi) {
+    const int n = (int)bi.prep.size(), T = force_tile(bi.opt);
+    std::vector<int32_t> list;
+    std::vector<int2> hc, el, nd;
+    for (int i = 0; i < n; ++i) {
+        if (bi.host_done[i]) continue;
+        list.push_back(i);
+        const Prepared& P = bi.prep[i];
+        for (int s = 0; s < P.n_hc; s += T) hc.push_back(make_int2(i, s));
+        for (int s = P.n_hc; s < P.n_elems; s += T) el.push_back(make_int2(i, s));
+        for (int k = 0; k < P.n_nodes; k += NODE_TILE) nd.push_back(make_int2(i, k));
+    }
+    auto put = [&](void* dst, const void* src, size_t bytes) {
+        if (bytes) HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, bi.stream));
+    };
+    put(bi.d_list.p, list.data(), list.size() * sizeof(int32_t));
+    put(bi.d_tiles_hc.p, hc.data(), hc.size() * sizeof(int2));
+    put(bi.d_tiles_el.p, el.data(), el.size() * sizeof(int2));
+    put(bi.d_tiles_node.p, nd.data(), nd.size() * sizeof(int2));
+    const int32_t counts[4] = {(int32_t)list.size(), (int32_t)hc.size(), (int32_t)el.size(), (int32_t)nd.size()};
+    put(bi.d_counts.p, counts, sizeof counts);
+    HIP_OK(hipStreamSynchronize(bi.stream));
+    bi.listed = counts[0];
+    bi.n_hc = counts[1];
+    bi.n_el = counts[2];
+    bi.n_node = counts[3];
+}
+
+template <int B, int T>
+static void launch_step(BatchImpl& bi, double cs, hipEvent_t* ev) {
+    const int32_t* cnt = bi.d_counts.p;
+    if (ev) HIP_OK(hipEventRecord(ev[0], bi.stream));
+    hipLaunchKernelGGL(k_inst<B>, dim3(std::max(bi.cap_inst, 1)), dim3(B), 0, bi.stream, bi.table.p, bi.d_list.p, cnt,
+                       cs, T, bi.done_flag.p, bi.n_done.p);
+    if (ev) HIP_OK(hipEventRecord(ev[1], bi.stream));
+    if (bi.cap_node > 0)
+        hipLaunchKernelGGL(k_half_tiles<NODE_TILE>, dim3(bi.cap_node), dim3(NODE_TILE), 0, bi.stream, bi.table.p,
+                           bi.d_tiles_node.p, cnt + 3);
+    if (ev) HIP_OK(hipEventRecord(ev[2], bi.stream));
+    if (bi.cap_hc > 0)
+        hipLaunchKernelGGL((k_force_tiles<CRUSH_MAT_HONEYCOMB, T>), dim3(bi.cap_hc), dim3(T), 0, bi.stream, bi.table.p,
+                           bi.d_tiles_hc.p, cnt + 1);
+    if (ev) HIP_OK(hipEventRecord(ev[3], bi.stream));
+    if (bi.cap_el > 0)
+        hipLaunchKernelGGL((k_force_tiles<CRUSH_MAT_ELASTIC, T>), dim3(bi.cap_el), dim3(T), 0, bi.stream, bi.table.p,
+                           bi.d_tiles_el.p, cnt + 2);
+    if (ev) HIP_OK(hipEventRecord(ev[4], bi.stream));
+    if (bi.cap_node > 0)
+        hipLaunchKernelGGL(k_gather_tiles<NODE_TILE>, dim3(bi.cap_node), dim3(NODE_TILE), 0, bi.stream, bi.table.p,
+                           bi.d_tiles_node.p, cnt + 3);
+    if (ev) HIP_OK(hipEventRecord(ev[5], bi.stream));
+}
+
+template <int T>
+static void launch_dt_init(BatchImpl& bi) {
+    if (bi.n_hc > 0)
+        hipLaunchKernelGGL((k_dt_tiles<CRUSH_MAT_HONEYCOMB, T>), dim3(bi.n_hc), dim3(T), 0, bi.stream, bi.table.p,
+                           bi.d_tiles_hc.p, bi.d_counts.p + 1);
+    if (bi.n_el > 0)
+        hipLaunchKernelGGL((k_dt_tiles<CRUSH_MAT_ELASTIC, T>), dim3(bi.n_el), dim3(T), 0, bi.stream, bi.table.p,
+                           bi.d_tiles_el.p, bi.d_counts.p + 2);
 }
 
 template <int B>
-static void launch_batch(BatchImpl& bi, int n) {
-    hipLaunchKernelGGL(k_queue_reset, dim3(1), dim3(1), 0, bi.stream, bi.counter.p);
-    const int S = bi.opt.steps_per_launch > 0 ? bi.opt.steps_per_launch : 0x7fffffff;
+static void launch_step_t(BatchImpl& bi, double cs, hipEvent_t* ev) {
+    const int T = force_tile(bi.opt);
+    if (T == 64)
+        launch_step<B, 64>(bi, cs, ev);
+    else if (T == 128)
+        launch_step<B, 128>(bi, cs, ev);
+    else
+        launch_step<B, 256>(bi, cs, ev);
+}
+
+// One step's launches; ev (EV_PER_STEP events, or null) brackets the kernels.
+static void launch_one(BatchImpl& bi, hipEvent_t* ev) {
     const double cs = bi.opt.contact_cell_scale > 0.0 ? bi.opt.contact_cell_scale : 1.0;
-    hipLaunchKernelGGL(k_batch<B>, dim3(bi.grid), dim3(B), 0, bi.stream, bi.table.p, bi.d_order.p, n, S, cs,
-                       bi.counter.p, bi.n_done.p);
+    const int B = inst_block(bi.opt);
+    if (B == 64)
+        launch_step_t<64>(bi, cs, ev);
+    else if (B == 128)
+        launch_step_t<128>(bi, cs, ev);
+    else if (B == 512)
+        launch_step_t<512>(bi, cs, ev);
+    else
+        launch_step_t<256>(bi, cs, ev);
 }
 
-template <int B>
-static int occupancy_grid() {
-    int dev = 0, cus = 0, per_cu = 0;
-    HIP_OK(hipGetDevice(&dev));
-    HIP_OK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-    HIP_OK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(k_batch<B>), B, 0));
-    return std::max(1, cus * std::max(per_cu, 1));
+// Add the kernel durations of the last replay.
+static void read_kernel_times(BatchImpl& bi) {
+    if (!bi.time_kernels) return;
+    double* acc[5] = {&bi.t_inst, &bi.t_half, &bi.t_hc, &bi.t_el, &bi.t_gather};
+    for (int r = 0; r < steps_per_graph(bi.opt); ++r)
+        for (int k = 0; k < 5; ++k) {
+            float ms = 0.0f;
+            HIP_OK(hipEventElapsedTime(&ms, bi.ev[EV_PER_STEP * r + k], bi.ev[EV_PER_STEP * r + k + 1]));
+            *acc[k] += ms;
+        }
+}
+
+// Capture steps_per_graph steps with grids sized to the current lists.
+static void capture(BatchImpl& bi) {
+    if (bi.graph) {
+        HIP_OK(hipGraphExecDestroy(bi.graph));
+        bi.graph = nullptr;
+    }
+    bi.cap_inst = bi.listed;
+    bi.cap_hc = bi.n_hc;
+    bi.cap_el = bi.n_el;
+    bi.cap_node = bi.n_node;
+    const int R = steps_per_graph(bi.opt);
+    hipGraph_t g;
+    HIP_OK(hipStreamBeginCapture(bi.stream, hipStreamCaptureModeThreadLocal));
+    for (int r = 0; r < R; ++r) launch_one(bi, nullptr);
+    HIP_OK(hipStreamEndCapture(bi.stream, &g));
+    HIP_OK(hipGraphInstantiate(&bi.graph, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+}
+
+// The launches of one graph replay: the graph, or with kernel timing on,
+// the same steps launched directly with events around each kernel.
+static void replay(BatchImpl& bi) {
+    const int R = steps_per_graph(bi.opt);
+    if (bi.fresh) {
+        // stable-step partials of the first step
+        const int T = force_tile(bi.opt);
+        if (T == 64)
+            launch_dt_init<64>(bi);
+        else if (T == 128)
+            launch_dt_init<128>(bi);
+        else
+            launch_dt_init<256>(bi);
+        bi.fresh = false;
+    }
+    if (!bi.time_kernels) {
+        HIP_OK(hipGraphLaunch(bi.graph, bi.stream));
+        return;
+    }
+    while ((int)bi.ev.size() < EV_PER_STEP * R) {
+        hipEvent_t e;
+        HIP_OK(hipEventCreate(&e));
+        bi.ev.push_back(e);
+    }
+    for (int r = 0; r < R; ++r) launch_one(bi, &bi.ev[EV_PER_STEP * (size_t)r]);
 }
 
 void batch_defaults(crush_batch_options* o) {
     o->threads_per_instance = 256;
-    o->steps_per_launch = 0;
-    o->launches_per_graph = 1;
+    o->force_tile = 256;
+    o->steps_per_graph = 32;
     o->host_threads = 0;
     o->contact_cell_scale = 1.0;
+    o->autotune = 0;
+    o->reserved0 = 0;
 }
+
+static void upload_images(BatchImpl& bi) {
+    const int n = (int)bi.prep.size();
+    for (int i = 0; i < n; ++i) {
+        std::vector<uint8_t> img;
+        build_image(bi.prep[i], img);
+        HIP_OK(hipMemcpyAsync(bi.pool.p + bi.base_off[i], img.data(), img.size(), hipMemcpyHostToDevice, bi.stream));
+        HIP_OK(hipStreamSynchronize(bi.stream));
+    }
+}
+
+// Restore the initial state and the full active lists.
+static void reset_state(BatchImpl& bi, bool recapture) {
+    upload_images(bi);
+    bi.n_done.zero(bi.stream);
+    bi.done_flag.zero(bi.stream);
+    std::fill(bi.host_done.begin(), bi.host_done.end(), 0);
+    rebuild_lists(bi);
+    if (recapture || bi.cap_inst != bi.listed || bi.cap_hc != bi.n_hc || bi.cap_el != bi.n_el ||
+        bi.cap_node != bi.n_node)
+        capture(bi);
+    bi.fresh = true;
+    bi.summary_valid = false;
+}
+
+static void autotune(BatchImpl& bi);
 
 BatchImpl* batch_create(const crush_model* const* models, int n, const crush_batch_options* opt) {
     require_device();
@@ -476,7 +730,6 @@ BatchImpl* batch_create(const crush_model* const* models, int n, const crush_bat
     auto bi = std::make_unique<BatchImpl>();
     batch_defaults(&bi->opt);
     if (opt) bi->opt = *opt;
-    if (bi->opt.launches_per_graph < 1) bi->opt.launches_per_graph = 1;
     auto t0 = std::chrono::steady_clock::now();
     bi->prep.resize(n);
     bi->offs.resize(n);
@@ -512,65 +765,96 @@ BatchImpl* batch_create(const crush_model* const* models, int n, const crush_bat
     bi->prepare_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     // pool layout
     bi->base_off.resize(n);
-    size_t total = 0;
+    size_t total = 0, slots = 0;
+    std::vector<size_t> dtp_off(n);
     for (int i = 0; i < n; ++i) {
         bi->base_off[i] = total;
         total += (bi->offs[i].total + 255) & ~size_t(255);
+        dtp_off[i] = slots;
+        slots += dtp_slots(bi->prep[i]);
     }
     HIP_OK(hipStreamCreateWithFlags(&bi->stream, hipStreamNonBlocking));
     bi->pool.alloc(total);
+    bi->dtp.alloc(2 * slots);
     std::vector<InstDev> tab(n);
     const uintptr_t base = reinterpret_cast<uintptr_t>(bi->pool.p);
     for (int i = 0; i < n; ++i) {
         tab[i] = make_inst(bi->prep[i], bi->offs[i], base + bi->base_off[i]);
+        tab[i].dtp_min = bi->dtp.p + dtp_off[i];
+        tab[i].dtp_sum = bi->dtp.p + slots + dtp_off[i];
         HIP_OK(hipMemcpyAsync(bi->pool.p + bi->base_off[i], images[i].data(), images[i].size(), hipMemcpyHostToDevice,
                               bi->stream));
     }
     bi->table.upload(tab, bi->stream);
-    // largest instances first (longest-processing-time order)
-    bi->order.resize(n);
-    std::iota(bi->order.begin(), bi->order.end(), 0);
-    std::vector<double> cost(n);
-    for (int i = 0; i < n; ++i)
-        cost[i] = (double)bi->prep[i].n_elems * bi->prep[i].settings.end_time / std::max(bi->prep[i].dt0, 1e-12);
-    std::stable_sort(bi->order.begin(), bi->order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-    bi->d_order.upload(bi->order, bi->stream);
-    bi->counter.alloc(1);
+    // list buffers sized for the smallest tile
+    size_t tiles_hc = 0, tiles_el = 0, tiles_node = 0;
+    for (int i = 0; i < n; ++i) {
+        tiles_hc += (bi->prep[i].n_hc + 63) / 64;
+        tiles_el += (bi->prep[i].n_elems - bi->prep[i].n_hc + 63) / 64;
+        tiles_node += (bi->prep[i].n_nodes + NODE_TILE - 1) / NODE_TILE;
+    }
+    bi->d_list.alloc(n);
+    bi->d_tiles_hc.alloc(std::max<size_t>(tiles_hc, 1));
+    bi->d_tiles_el.alloc(std::max<size_t>(tiles_el, 1));
+    bi->d_tiles_node.alloc(std::max<size_t>(tiles_node, 1));
+    bi->d_counts.alloc(4);
     bi->n_done.alloc(1);
     bi->n_done.zero(bi->stream);
+    bi->done_flag.alloc(n);
+    bi->done_flag.zero(bi->stream);
+    bi->host_done.assign(n, 0);
     bi->summary.alloc(n);
+    bi->t_start.alloc(1);
+    {
+        int dev = 0, khz = 0;
+        HIP_OK(hipGetDevice(&dev));
+        HIP_OK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+        bi->clock_khz = khz > 0 ? (double)khz : 1.0;
+    }
     HIP_OK(hipStreamSynchronize(bi->stream));
     bi->upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
-    const int B = block_of(bi->opt);
-    bi->grid = B == 1024 ? occupancy_grid<1024>() : (B == 512 ? occupancy_grid<512>() : occupancy_grid<256>());
-    // capture the launch sequence
-    hipGraph_t g;
-    HIP_OK(hipStreamBeginCapture(bi->stream, hipStreamCaptureModeThreadLocal));
-    for (int r = 0; r < bi->opt.launches_per_graph; ++r) {
-        if (B == 1024)
-            launch_batch<1024>(*bi, n);
-        else if (B == 512)
-            launch_batch<512>(*bi, n);
-        else
-            launch_batch<256>(*bi, n);
-    }
-    HIP_OK(hipStreamEndCapture(bi->stream, &g));
-    HIP_OK(hipGraphInstantiate(&bi->graph, g, nullptr, nullptr, 0));
-    (void)hipGraphDestroy(g);
+    rebuild_lists(*bi);
+    capture(*bi);
+    bi->fresh = true;
+    if (bi->opt.autotune) autotune(*bi);
     return bi.release();
 }
 
-void batch_reset(BatchImpl& bi) {
+void batch_reset(BatchImpl& bi) { reset_state(bi, false); }
+
+// Milliseconds from the start of the last crush_batch_run to the retirement
+// of instance i (device wall clock).
+static double retire_ms(const BatchImpl& bi, int i) {
+    const unsigned long long tr = bi.host_summary[i].t_retire;
+    return tr > bi.host_t_start ? (double)(tr - bi.host_t_start) / bi.clock_khz : 0.0;
+}
+
+// Replays the step graph until every instance has retired (or, when
+// max_replays > 0, at most that many replays).  Between replays the active
+// lists drop retired instances once a quarter of the listed ones have
+// retired, and the graph is captured again with smaller grids once the
+// lists have shrunk to half of what it was captured for.
+static void drive(BatchImpl& bi, long long max_replays) {
     const int n = (int)bi.prep.size();
-    for (int i = 0; i < n; ++i) {
-        std::vector<uint8_t> img;
-        build_image(bi.prep[i], img);
-        HIP_OK(hipMemcpyAsync(bi.pool.p + bi.base_off[i], img.data(), img.size(), hipMemcpyHostToDevice, bi.stream));
+    int done = 0;
+    long long guard = 0;
+    while (done < n) {
+        replay(bi);
+        bi.launches += steps_per_graph(bi.opt);
+        bi.replays += 1;
+        HIP_OK(hipMemcpyAsync(&done, bi.n_done.p, sizeof(int), hipMemcpyDeviceToHost, bi.stream));
         HIP_OK(hipStreamSynchronize(bi.stream));
+        read_kernel_times(bi);
+        if (max_replays > 0 && bi.replays >= max_replays) break;
+        if (++guard > 100000000LL) fail(CRUSH_E_INPUT, "batch did not finish");
+        const int active = n - done;
+        if (active > 0 && 4 * (bi.listed - active) >= bi.listed) {
+            HIP_OK(hipMemcpyAsync(bi.host_done.data(), bi.done_flag.p, n, hipMemcpyDeviceToHost, bi.stream));
+            HIP_OK(hipStreamSynchronize(bi.stream));
+            rebuild_lists(bi);
+            if (2 * bi.listed <= bi.cap_inst) capture(bi);
+        }
     }
-    bi.n_done.zero(bi.stream);
-    HIP_OK(hipStreamSynchronize(bi.stream));
-    bi.summary_valid = false;
 }
 
 void batch_run(BatchImpl& bi, crush_batch_stats* st) {
@@ -580,16 +864,11 @@ void batch_run(BatchImpl& bi, crush_batch_stats* st) {
     HIP_OK(hipEventCreate(&e0));
     HIP_OK(hipEventCreate(&e1));
     HIP_OK(hipEventRecord(e0, bi.stream));
-    int done = 0;
-    long long guard = 0;
-    while (done < n) {
-        HIP_OK(hipGraphLaunch(bi.graph, bi.stream));
-        bi.launches += bi.opt.launches_per_graph;
-        bi.replays += 1;
-        HIP_OK(hipMemcpyAsync(&done, bi.n_done.p, sizeof(int), hipMemcpyDeviceToHost, bi.stream));
-        HIP_OK(hipStreamSynchronize(bi.stream));
-        if (++guard > 100000000LL) fail(CRUSH_E_INPUT, "batch did not finish");
-    }
+    hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, bi.stream, bi.t_start.p);
+    bi.launches = 0;
+    bi.replays = 0;
+    bi.t_inst = bi.t_half = bi.t_hc = bi.t_el = bi.t_gather = 0.0;
+    drive(bi, 0);
     HIP_OK(hipEventRecord(e1, bi.stream));
     HIP_OK(hipEventSynchronize(e1));
     float ms = 0.0f;
@@ -600,6 +879,8 @@ void batch_run(BatchImpl& bi, crush_batch_stats* st) {
     bi.host_summary.resize(n);
     HIP_OK(hipMemcpyAsync(bi.host_summary.data(), bi.summary.p, n * sizeof(BatchSummary), hipMemcpyDeviceToHost,
                           bi.stream));
+    HIP_OK(hipMemcpyAsync(&bi.host_t_start, bi.t_start.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                          bi.stream));
     HIP_OK(hipStreamSynchronize(bi.stream));
     bi.summary_valid = true;
     if (st) {
@@ -608,18 +889,83 @@ void batch_run(BatchImpl& bi, crush_batch_stats* st) {
         st->device_ms = ms;
         st->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
         st->instances = n;
-        long long es = 0;
-        for (int i = 0; i < n; ++i) es += (long long)bi.prep[i].n_elems * bi.host_summary[i].steps;
+        long long es = 0, hc = 0;
+        for (int i = 0; i < n; ++i) {
+            es += (long long)bi.prep[i].n_elems * bi.host_summary[i].steps;
+            hc += (long long)bi.prep[i].n_hc * bi.host_summary[i].steps;
+        }
         st->element_steps = es;
+        st->hc_element_steps = hc;
         st->launches = bi.launches;
         st->graph_replays = bi.replays;
         st->pool_bytes = (double)bi.pool.n;
+        std::vector<double> lat(n);
+        for (int i = 0; i < n; ++i) lat[i] = retire_ms(bi, i);
+        std::sort(lat.begin(), lat.end());
+        auto pct = [&](double q) { return lat[std::min<size_t>(n - 1, (size_t)std::ceil(q * n) - 1)]; };
+        st->latency_p50_ms = pct(0.50);
+        st->latency_p99_ms = pct(0.99);
+        const bool tk = bi.time_kernels;
+        st->k_inst_ms = tk ? bi.t_inst : 0.0;
+        st->k_node_ms = tk ? bi.t_half + bi.t_gather : 0.0;
+        st->k_force_hc_ms = tk ? bi.t_hc : 0.0;
+        st->k_force_el_ms = tk ? bi.t_el : 0.0;
     }
     for (int i = 0; i < n; ++i)
         if (bi.host_summary[i].error)
             fail(CRUSH_E_NONFINITE, "instance %d: displacement became non-finite at step %lld", i,
                  bi.host_summary[i].steps);
 }
+
+// Timed trials over the knobs that do not change results (workgroup size
+// of the per-instance kernel, force tile, contact grid cell scale): each
+// candidate runs a few graph replays from the initial state; the fastest
+// is kept and the state restored.
+static void autotune(BatchImpl& bi) {
+    const crush_batch_options base = bi.opt;
+    struct Cand {
+        int b, t;
+        double cs;
+    };
+    const Cand cands[] = {{256, 256, 1.0}, {128, 256, 1.0}, {512, 256, 1.0}, {256, 128, 1.0},
+                          {256, 64, 1.0},  {256, 256, 0.5}, {256, 256, 2.0}};
+    double best = INFINITY;
+    crush_batch_options best_opt = base;
+    const long long trial_replays = 3;
+    for (const Cand& cd : cands) {
+        bi.opt = base;
+        bi.opt.threads_per_instance = cd.b;
+        bi.opt.force_tile = cd.t;
+        bi.opt.contact_cell_scale = cd.cs;
+        reset_state(bi, true);
+        // one replay to warm up, then time the rest
+        bi.replays = 0;
+        drive(bi, 1);
+        hipEvent_t e0, e1;
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        HIP_OK(hipEventRecord(e0, bi.stream));
+        bi.replays = 0;
+        drive(bi, trial_replays);
+        HIP_OK(hipEventRecord(e1, bi.stream));
+        HIP_OK(hipEventSynchronize(e1));
+        float ms = 0.0f;
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (ms < best) {
+            best = ms;
+            best_opt = bi.opt;
+        }
+    }
+    bi.opt = best_opt;
+    bi.opt.autotune = 0;
+    reset_state(bi, true);
+}
+
+void batch_options_of(const BatchImpl& bi, crush_batch_options* o) { *o = bi.opt; }
+
+void batch_time_kernels(BatchImpl& bi, bool on) { bi.time_kernels = on; }
 
 void batch_result(BatchImpl& bi, int i, crush_results* out) {
     if (i < 0 || i >= (int)bi.prep.size()) fail(CRUSH_E_INPUT, "instance %d out of range", i);
@@ -634,7 +980,7 @@ void batch_result(BatchImpl& bi, int i, crush_results* out) {
     out->added_mass_max = r.added_max;
     out->separated = r.separated;
     out->wall_ms = 0.0;
-    out->device_ms = 0.0;
+    out->device_ms = retire_ms(bi, i);
     if (out->displacement)
         HIP_OK(hipMemcpy(out->displacement, base + o.u, sizeof(double) * 3 * P.n_nodes, hipMemcpyDeviceToHost));
     if (out->velocity)
@@ -682,6 +1028,20 @@ int32_t crush_batch_run(crush_batch* b, crush_batch_stats* stats) {
     return guarded([&] {
         if (!b) fail(CRUSH_E_INPUT, "null batch");
         batch_run(*b->impl, stats);
+    });
+}
+
+int32_t crush_batch_options_get(const crush_batch* b, crush_batch_options* opt) {
+    return guarded([&] {
+        if (!b || !opt) fail(CRUSH_E_INPUT, "null argument");
+        batch_options_of(*b->impl, opt);
+    });
+}
+
+int32_t crush_batch_time_kernels(crush_batch* b, int32_t on) {
+    return guarded([&] {
+        if (!b) fail(CRUSH_E_INPUT, "null batch");
+        batch_time_kernels(*b->impl, on != 0);
     });
 }
 

@@ -18,6 +18,7 @@ struct BatchSummary {
     double dv[8][3];
     double crush_end[8];
     double crush_max[8];
+    unsigned long long t_retire;
 };
 
 struct BatchImpl;
@@ -28,5 +29,7 @@ void batch_run(BatchImpl& bi, crush_batch_stats* st);
 void batch_reset(BatchImpl& bi);
 void batch_result(BatchImpl& bi, int i, crush_results* out);
 void batch_free(BatchImpl* bi);
+void batch_options_of(const BatchImpl& bi, crush_batch_options* o);
+void batch_time_kernels(BatchImpl& bi, bool on);
 
 }  // namespace crush

@@ -34,9 +34,15 @@ struct StepCtl {
     unsigned long long eroded_total;
     int done, error, scale_now, had_pair;
     int quiet, separated, pair_hits, chk;
-    int sep_steps, adaptive, reserved0, reserved1;
+    int sep_steps, adaptive;
+    int pending;      // batched path: element forces of the open step are due
+    int eroded_step;  // batched path: elements eroded by the open step's force pass
+    int bad_step;     // batched path: a node of the open step became non-finite
+    int reserved3;
     double crush_max[8];
     double crush_end[8];
+    unsigned long long t_retire;  // device wall clock at retirement (batched path)
+    unsigned long long reserved2;
 };
 
 // Face geometry record width (doubles) in the SoA face array:
@@ -528,7 +534,7 @@ CRUSH_HD void face_geometry(const double (*x)[3], double max_d, double* out /* [
     const double cu[3] = {-C_TRI_TOL, 1.0 + 2.0 * C_TRI_TOL, -C_TRI_TOL};
     const double cv[3] = {-C_TRI_TOL, -C_TRI_TOL, 1.0 + 2.0 * C_TRI_TOL};
     double tlo[3] = {INFINITY, INFINITY, INFINITY}, thi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    bool valid = isfinite(nn) && nn > 0.0;
+    bool valid = __builtin_isfinite(nn) && nn > 0.0;
     for (int t = 0; t < 4 && valid; ++t) {
         const double* a = x[tri[t][0]];
         const double* b = x[tri[t][1]];

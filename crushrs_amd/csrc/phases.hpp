@@ -90,6 +90,10 @@ struct InstDev {
     const double* g_v0;   // [3G] initial mean velocities
     double* g_dv;         // [3G] delta-v written when the solve ends
     StepCtl* ctl;
+    // batched path: per-tile partials of the stable step (min dt_e, sum of
+    // scaling masses), one slot per force tile of the instance
+    double* dtp_min;
+    double* dtp_sum;
 };
 
 __device__ __forceinline__ double step_dt(const StepCtl& c) { return fmax(fmin(c.dt_cfl, c.end_time - c.t), 1e-15); }

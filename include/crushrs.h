@@ -201,11 +201,14 @@ int32_t crush_contact_node_count(const crush_model* model, int32_t* count);
 typedef struct crush_batch crush_batch;
 
 typedef struct crush_batch_options {
-    int32_t threads_per_instance; /* workgroup size per instance: 256, 512 or 1024 (0: 256) */
-    int32_t steps_per_launch;     /* steps per instance per launch (0: to completion) */
-    int32_t launches_per_graph;   /* launches captured in one hipGraph (>= 1) */
+    int32_t threads_per_instance; /* workgroup of the per-instance step kernel: 64, 128, 256 or 512 (0: 256) */
+    int32_t force_tile;           /* element slots per force workgroup: 64, 128 or 256 (0: 256) */
+    int32_t steps_per_graph;      /* explicit steps captured in one hipGraph (0: 32) */
     int32_t host_threads;         /* host threads preparing models (0: all cores, max 16) */
     double contact_cell_scale;    /* contact grid cell / largest face extent (0: 1.0) */
+    int32_t autotune;             /* 1: choose the three knobs above that do not change
+                                     results by timed trials at create time */
+    int32_t reserved0;
 } crush_batch_options;
 
 typedef struct crush_batch_stats {
@@ -215,9 +218,16 @@ typedef struct crush_batch_stats {
     double wall_ms;               /* host wall time of crush_batch_run        */
     int64_t instances;
     int64_t element_steps;        /* sum over instances of hexes x steps      */
-    int64_t launches;             /* kernel launches of the batched step      */
-    int64_t graph_replays;
+    int64_t launches;             /* explicit steps launched (last run)       */
+    int64_t graph_replays;        /* hipGraph replays (last run)              */
     double pool_bytes;            /* device memory of the instance images     */
+    double latency_p50_ms;        /* median time from run start to retirement */
+    double latency_p99_ms;        /* 99th percentile of the same              */
+    int64_t hc_element_steps;     /* honeycomb part of element_steps          */
+    double k_inst_ms;             /* with kernel timing on, summed durations: */
+    double k_node_ms;             /*   per-instance kernel, node kernels      */
+    double k_force_hc_ms;         /*   (half + gather), honeycomb and elastic */
+    double k_force_el_ms;         /*   force kernels                          */
 } crush_batch_stats;
 
 /* Fill opt with the defaults. */
@@ -230,12 +240,22 @@ int32_t crush_batch_create(const crush_model* const* models, int32_t n, const cr
 /* Solve every instance to its end (end_time or separation). */
 int32_t crush_batch_run(crush_batch* b, crush_batch_stats* stats);
 
+/* Options in effect (after autotuning). */
+int32_t crush_batch_options_get(const crush_batch* b, crush_batch_options* opt);
+
+/* Kernel timing on (1) or off (0): with it on, crush_batch_run launches
+ * the steps directly instead of replaying the hipGraph, records HIP events
+ * around each kernel and reports the summed kernel durations in its stats. */
+int32_t crush_batch_time_kernels(crush_batch* b, int32_t on);
+
 /* Restore every instance to its initial state (so the batch can be run
  * again, e.g. for timing). */
 int32_t crush_batch_reset(crush_batch* b);
 
 /* Results of instance i (fields as crush_run; the displacement and
- * velocity arrays are copied only when the pointers are non-NULL). */
+ * velocity arrays are copied only when the pointers are non-NULL).
+ * device_ms is the time from the start of the last crush_batch_run to the
+ * retirement of this instance; wall_ms is 0. */
 int32_t crush_batch_result(crush_batch* b, int32_t i, crush_results* out);
 
 void crush_batch_free(crush_batch* b);

@@ -45,12 +45,12 @@ def same_as_plain(br, pr, tol=1e-9):
         assert abs(a.crush_max - b.crush_max) <= tol
 
 
-@pytest.mark.parametrize("tpi,spl", [(256, 0), (512, 7), (1024, 64)])
-def test_batch_matches_plain(tpi, spl):
-    print(f"case: batch mix threads_per_instance={tpi} steps_per_launch={spl}", flush=True)
+@pytest.mark.parametrize("tpi,tile,spg", [(256, 256, 32), (64, 64, 7), (512, 128, 1), (128, 256, 100)])
+def test_batch_matches_plain(tpi, tile, spg):
+    print(f"case: batch mix threads_per_instance={tpi} force_tile={tile} steps_per_graph={spg}", flush=True)
     models = mix()
     plain = [ca.run(m) for m in models]
-    b = ca.Batch(models, threads_per_instance=tpi, steps_per_launch=spl, launches_per_graph=2)
+    b = ca.Batch(models, threads_per_instance=tpi, force_tile=tile, steps_per_graph=spg)
     st = b.run()
     assert st.instances == len(models)
     for i, p in enumerate(plain):
@@ -60,10 +60,10 @@ def test_batch_matches_plain(tpi, spl):
 def test_batch_alone_vs_mix_and_reset():
     print("case: batch alone vs mix, reset", flush=True)
     models = mix()
-    alone = ca.Batch([models[1]], steps_per_launch=16)
+    alone = ca.Batch([models[1]], steps_per_graph=16)
     alone.run()
     ra = alone.result(0)
-    crowd = ca.Batch(models + [ca.Model.crash(25.0, 25.0) for _ in range(11)], steps_per_launch=16)
+    crowd = ca.Batch(models + [ca.Model.crash(25.0, 25.0) for _ in range(11)], steps_per_graph=16)
     crowd.run()
     rm = crowd.result(1)
     assert np.array_equal(ra.displacement, rm.displacement)
@@ -77,6 +77,21 @@ def test_batch_alone_vs_mix_and_reset():
     crowd.run()
     rr = crowd.result(1)
     assert np.array_equal(rr.displacement, rm.displacement)
+
+
+def test_batch_autotune_keeps_results():
+    print("case: batch autotune", flush=True)
+    models = mix()[:3]
+    ref = ca.Batch(models)
+    ref.run()
+    tuned = ca.Batch(models, autotune=1)
+    o = tuned.options
+    assert o.autotune == 0 and o.threads_per_instance in (64, 128, 256, 512) and o.force_tile in (64, 128, 256)
+    tuned.run()
+    for i in range(len(models)):
+        a, b = ref.result(i), tuned.result(i)
+        assert a.steps == b.steps
+        assert np.array_equal(a.displacement, b.displacement)
 
 
 def test_batch_vs_oracle_delta_v():
