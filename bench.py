@@ -48,6 +48,8 @@ HBM_PEAK_GBS = 8000.0
 # not counted.
 B_HC, B_EL, B_NODE_FORCE, B_NODE_UPDATE = 776, 576, 36, 76
 B_DT_ELEM, B_DT_NODE = 36, 24
+KERNELS = ("k_end", "k_grid", "k_search_tiles", "k_faceforce_tiles", "k_apply_tiles", "k_half_tiles",
+           "k_force_tiles<honeycomb>", "k_force_tiles<elastic>", "k_gather_tiles")
 
 
 def step_bytes(n_hc: int, n_el: int, n_nodes: int, steps: int, check_every: int = 5) -> float:
@@ -132,7 +134,7 @@ def plain_gpu(models, k: int = 8):
 
 def bench_sweep(args):
     t0 = time.perf_counter()
-    cases = workloads.headon_sweep(args.instances, seed=args.seed)
+    cases = workloads.headon_sweep(args.instances, seed=args.seed, end_time=args.end_time)
     models = workloads.build_models(cases)
     t_build = time.perf_counter() - t0
     n_hc = np.array([workloads.honeycomb_count(m) for m in models])
@@ -141,6 +143,8 @@ def bench_sweep(args):
     opt = ca.batch_options(threads_per_instance=args.threads_per_instance, force_tile=args.force_tile,
                            steps_per_graph=args.steps_per_graph, autotune=int(args.autotune))
     batch = ca.Batch(models, opt)
+    if args.eager:
+        batch.time_kernels(True)
     for _ in range(args.warmup):
         batch.run()
         batch.reset()
@@ -163,18 +167,20 @@ def bench_sweep(args):
     # roofline of the dominant kernel, the honeycomb element-force tiles:
     # a second solve of the same sweep with HIP events recorded around every
     # launch (on the stream the kernels run on) gives the summed duration
-    batch.time_kernels(True)
-    batch.reset()
-    tk = batch.run()
-    batch.time_kernels(False)
+    if args.eager:
+        tk = st
+    else:
+        batch.time_kernels(True)
+        batch.reset()
+        tk = batch.run()
+        batch.time_kernels(False)
     k_launches = max(1, tk.launches)
     avg_ms = tk.k_force_hc_ms / k_launches
     bytes_per_launch = B_HC * tk.hc_element_steps / k_launches
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     el_steps = tk.element_steps - tk.hc_element_steps
     kernels = {
-        "k_inst_ms": tk.k_inst_ms, "k_node_ms": tk.k_node_ms, "k_force_hc_ms": tk.k_force_hc_ms,
-        "k_force_el_ms": tk.k_force_el_ms,
+        "ms": {name: tk.k_ms[j] for j, name in enumerate(KERNELS)},
         "launches_each": tk.launches,
         "force_el_gbs": B_EL * el_steps / max(tk.k_force_el_ms, 1e-9) / 1e6,
         "whole_step_algorithmic_gbs": total_bytes / (tk.device_ms * 1e-3) / 1e9,
@@ -194,7 +200,7 @@ def bench_sweep(args):
         "data": "synthetic: seeded head-on/offset pairs of the reference's four vehicle TOMLs, generated meshes",
         "config": {
             "workload": (f"batched sweep: {n} head-on/offset pairs (neon/silverado/expedition/navigator TOMLs), "
-                         "speeds U(10,40) mph, offset U(0,0.6) m, h in {0.3,0.2} m, rigid road, 0.15 s"),
+                         f"speeds U(10,40) mph, offset U(0,0.6) m, h in {{0.3,0.2}} m, rigid road, {args.end_time} s"),
             "instances": n, "seed": args.seed,
             "elements": {"min": int(E.min()), "max": int(E.max()), "mean": float(E.mean())},
             "solver_steps": {"min": int(steps.min()), "max": int(steps.max()), "mean": float(steps.mean())},
@@ -274,6 +280,13 @@ def main():
     ap.add_argument("--force-tile", type=int, default=256)
     ap.add_argument("--steps-per-graph", type=int, default=32)
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--end-time", type=float, default=0.15,
+                    help="simulated time per instance (0.15 s; shorter only for profiling runs)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device-to-device copy of BYTES (HBM counter calibration)")
+    ap.add_argument("--eager", action="store_true",
+                    help="launch every step directly with kernel timing (no hipGraph replay), e.g. under a "
+                         "kernel-trace profiler")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
@@ -284,6 +297,15 @@ def main():
     if ca.device_count() < 1:
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(1)
+    if args.copy_probe:
+        import torch
+        a = torch.ones(args.copy_probe // 4, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        torch.cuda.synchronize()
+        b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": args.copy_probe}))
+        return
     if args.workload == "sweep":
         args.steps = 1 if args.steps is None else args.steps
         args.warmup = 1 if args.warmup is None else args.warmup

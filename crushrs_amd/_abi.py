@@ -198,10 +198,12 @@ class BatchStats(C.Structure):
         ("latency_p50_ms", C.c_double),
         ("latency_p99_ms", C.c_double),
         ("hc_element_steps", C.c_int64),
-        ("k_inst_ms", C.c_double),
+        ("k_end_ms", C.c_double),
+        ("k_contact_ms", C.c_double),
         ("k_node_ms", C.c_double),
         ("k_force_hc_ms", C.c_double),
         ("k_force_el_ms", C.c_double),
+        ("k_ms", C.c_double * 12),
     ]
 
 

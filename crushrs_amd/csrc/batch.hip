@@ -36,6 +36,7 @@ namespace crush {
 
 constexpr int BATCH_GRID_CELLS = 1024;
 constexpr int BATCH_GRID_ENTRIES = 4096;
+constexpr int GRID_STRIDE = BATCH_GRID_CELLS + 1 + BATCH_GRID_ENTRIES;
 
 // ------------------------------------------------------------- images
 
@@ -71,7 +72,7 @@ struct ImageOffsets {
     size_t cdesc, cn_node, cn_scale, cf_nodes, cf_contact, tn_node, tn_ptr, tn_ent, rec_face, fh_count;
     size_t g_nodes_ptr, g_nodes, g_front_ptr, g_front, g_accel, g_axis, g_mass, g_v0, g_dv, ctl;
     size_t init_bytes;
-    size_t fe, fgeo, rec_f, fh_list, face_force;
+    size_t fe, fgeo, rec_f, fh_list, face_force, cgrid, cgp;
     size_t total;
 };
 
@@ -137,7 +138,7 @@ static ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     o.tn_node = b.put(P.tn_node);
     o.tn_ptr = b.put(P.tn_ptr);
     o.tn_ent = b.put(P.tn_ent);
-    o.rec_face = b.zeros(sizeof(int32_t) * std::max(P.n_cnodes, 1));
+    o.rec_face = b.put(std::vector<int32_t>(std::max(P.n_cnodes, 1), -1));
     o.fh_count = b.zeros(sizeof(int32_t) * std::max(P.n_cfaces, 1));
     o.g_nodes_ptr = b.put(P.g_nodes_ptr);
     o.g_nodes = b.put(P.g_nodes);
@@ -166,6 +167,8 @@ static ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     o.rec_f = scratch(sizeof(double) * 3 * (size_t)P.n_cnodes);
     o.fh_list = scratch(sizeof(int32_t) * FACE_HITS_CAP * (size_t)P.n_cfaces);
     o.face_force = scratch(sizeof(double) * 3 * (size_t)P.n_cfaces);
+    o.cgrid = scratch(sizeof(int32_t) * GRID_STRIDE * P.cdesc.size());
+    o.cgp = scratch(sizeof(FaceGrid) * P.cdesc.size());
     o.total = off;
     out.swap(b.bytes);
     return o;
@@ -234,36 +237,12 @@ static InstDev make_inst(const Prepared& P, const ImageOffsets& o, uintptr_t bas
     I.g_v0 = at<double>(base, o.g_v0);
     I.g_dv = at<double>(base, o.g_dv);
     I.ctl = at<StepCtl>(base, o.ctl);
+    I.cgrid = at<int32_t>(base, o.cgrid);
+    I.cgp = at<FaceGrid>(base, o.cgp);
     return I;
 }
 
 // ------------------------------------------------------------- kernels
-
-// Contact of one instance (src/contact.rs:92-160 for each contact in
-// order): face geometry, per-contact LDS grid and node search, face
-// reactions and the per-node sums.  Returns the number of penetrating
-// nodes of pair contacts seen by this thread.
-template <int B>
-__device__ __forceinline__ double ph_contact(const InstDev& D, double dt, double cell_scale, int* s_start,
-                                             int* s_cursor, int* s_entries, double* s_w, int* s_tmp) {
-    const int t = threadIdx.x;
-    double pair_hits = 0.0;
-    for (int f = t; f < D.F; f += B) contact_face(D, f);
-    __syncthreads();
-    for (int ci = 0; ci < D.n_contacts; ++ci) {
-        const ContactDesc cd = D.cdesc[ci];
-        const FaceGrid g = build_face_grid<B, BATCH_GRID_CELLS, BATCH_GRID_ENTRIES>(D, cd, cell_scale, s_start, s_cursor,
-                                                                                     s_entries, s_w, s_tmp);
-        for (int q = t; q < cd.n_nodes; q += B)
-            if (contact_node(D, cd, q, g, s_start, s_entries, dt) && cd.pair) pair_hits += 1.0;
-        __syncthreads();
-    }
-    for (int f = t; f < D.F; f += B) contact_face_force(D, f);
-    __syncthreads();
-    for (int i = t; i < D.T; i += B) contact_apply_node(D, i);
-    __syncthreads();
-    return pair_hits;
-}
 
 // Outputs of a retired instance: per-group delta-v and end crush.
 template <int B>
@@ -309,19 +288,14 @@ __device__ __forceinline__ void dt_partial(const InstDev& D, double dt_floor, in
 
 // Per-instance part of a step (src/solver.rs:287-352), one workgroup per
 // active instance.  It closes the open step (end-of-step bookkeeping after
-// the gather) and opens the next: the stable step from the tile partials
-// at check steps, then contact.  `pending` marks an open step whose half
-// step, element forces and gather are due (k_half_tiles, k_force_tiles,
-// k_gather_tiles).
+// the gather) and opens the next with its stable step (from the tile
+// partials, at check steps).  `pending` marks an open step whose contact,
+// half step, element forces and gather are due (the tile kernels below).
 template <int B>
-__global__ __launch_bounds__(B) void k_inst(const InstDev* __restrict__ table, const int32_t* __restrict__ list,
-                                            const int32_t* __restrict__ n_list, double cell_scale, int force_tile,
-                                            uint8_t* __restrict__ done_flag, int* __restrict__ n_done) {
-    __shared__ int s_start[BATCH_GRID_CELLS + 1];
-    __shared__ int s_cursor[BATCH_GRID_CELLS];
-    __shared__ int s_entries[BATCH_GRID_ENTRIES];
+__global__ __launch_bounds__(B) void k_end(const InstDev* __restrict__ table, const int32_t* __restrict__ list,
+                                           const int32_t* __restrict__ n_list, int force_tile,
+                                           uint8_t* __restrict__ done_flag, int* __restrict__ n_done) {
     __shared__ double s_w[B / 64];
-    __shared__ int s_tmp[B / 64];
     __shared__ StepCtl s_ctl;
     if ((int)blockIdx.x >= *n_list) return;
     const int inst = list[blockIdx.x];
@@ -340,6 +314,7 @@ __global__ __launch_bounds__(B) void k_inst(const InstDev* __restrict__ table, c
             c->eroded_total += (unsigned long long)c->eroded_step;
             c->eroded_step = 0;
             c->bad_step = 0;
+            c->pair_hits = 0;
             c->pending = 0;
         }
         __syncthreads();
@@ -351,40 +326,111 @@ __global__ __launch_bounds__(B) void k_inst(const InstDev* __restrict__ table, c
             }
         }
     }
-    if (!c->done) {
-        if (t == 0) {
-            c->scale_now = 0;
-            if (c->adaptive && c->step % c->chk == 0) {
-                // stable step and selective mass scaling (src/solver.rs:288-299)
-                const int T = force_tile;
-                const int slots = (D.n_hc + T - 1) / T + (D.E - D.n_hc + T - 1) / T;
-                double mn = INFINITY, sm = 0.0;
-                for (int j = 0; j < slots; ++j) {
-                    mn = fmin(mn, D.dtp_min[j]);
-                    sm += D.dtp_sum[j];
+    if (t == 0 && !c->done) {
+        c->scale_now = 0;
+        if (c->adaptive && c->step % c->chk == 0) {
+            // stable step and selective mass scaling (src/solver.rs:288-299)
+            const int T = force_tile;
+            const int slots = (D.n_hc + T - 1) / T + (D.E - D.n_hc + T - 1) / T;
+            double mn = INFINITY, sm = 0.0;
+            for (int j = 0; j < slots; ++j) {
+                mn = fmin(mn, D.dtp_min[j]);
+                sm += D.dtp_sum[j];
+            }
+            if (isfinite(mn)) {
+                const bool scale = c->dt_floor > 0.0 && mn < c->dt_floor;
+                if (scale) {
+                    c->added_total_max = fmax(c->added_total_max, sm);
+                    c->scale_now = 1;
                 }
-                if (isfinite(mn)) {
-                    const bool scale = c->dt_floor > 0.0 && mn < c->dt_floor;
-                    if (scale) {
-                        c->added_total_max = fmax(c->added_total_max, sm);
-                        c->scale_now = 1;
-                    }
-                    c->dt_cfl = c->dt_scale * fmin(scale ? c->dt_floor : mn, c->contact_dt);
-                }
+                c->dt_cfl = c->dt_scale * fmin(scale ? c->dt_floor : mn, c->contact_dt);
             }
         }
-        __syncthreads();
-        const double dt = step_dt(*c);
-        double hits = 0.0;
-        if (D.F > 0) hits = ph_contact<B>(D, dt, cell_scale, s_start, s_cursor, s_entries, s_w, s_tmp);
-        hits = block_reduce<B>(hits, 0, s_w);
-        if (t == 0) {
-            c->pair_hits = (int)hits;
-            c->pending = 1;
-        }
+        c->pending = 1;
     }
     __syncthreads();
     if (t == 0) *D.ctl = *c;
+}
+
+// Contact (src/contact.rs:92-160), in four launches over all open steps.
+// (1) geometry of the faces of one contact of one instance and its face
+// grid, built in LDS and stored to device memory.
+template <int B>
+__global__ __launch_bounds__(B) void k_grid(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
+                                            const int32_t* __restrict__ n_tiles, double cell_scale) {
+    __shared__ int s_start[BATCH_GRID_CELLS + 1];
+    __shared__ int s_cursor[BATCH_GRID_CELLS];
+    __shared__ int s_entries[BATCH_GRID_ENTRIES];
+    __shared__ double s_w[B / 64];
+    __shared__ int s_tmp[B / 64];
+    if ((int)blockIdx.x >= *n_tiles) return;
+    const int2 tile = tiles[blockIdx.x];
+    const InstDev& D = table[tile.x];
+    if (!D.ctl->pending) return;
+    const int ci = tile.y;
+    const ContactDesc cd = D.cdesc[ci];
+    for (int f = threadIdx.x; f < cd.n_faces; f += B) contact_face(D, cd.face_begin + f);
+    __syncthreads();
+    const FaceGrid g = build_face_grid<B, BATCH_GRID_CELLS, BATCH_GRID_ENTRIES>(D, cd, cell_scale, s_start, s_cursor,
+                                                                                 s_entries, s_w, s_tmp);
+    int32_t* gs = D.cgrid + (size_t)ci * GRID_STRIDE;
+    int32_t* ge = gs + BATCH_GRID_CELLS + 1;
+    if (!g.overflow) {
+        const int ncell = g.dim[0] * g.dim[1] * g.dim[2];
+        for (int i = threadIdx.x; i <= ncell; i += B) gs[i] = s_start[i];
+        const int total = s_start[ncell];
+        for (int i = threadIdx.x; i < total; i += B) ge[i] = s_entries[i];
+    }
+    if (threadIdx.x == 0) D.cgp[ci] = g;
+}
+
+// (2) node search, tiles of T contacting nodes of one contact of one
+// instance; penetrating nodes of pair contacts are counted per instance.
+template <int T>
+__global__ __launch_bounds__(T) void k_search_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
+                                                    const int32_t* __restrict__ n_tiles) {
+    if ((int)blockIdx.x >= *n_tiles) return;
+    const int2 tile = tiles[blockIdx.x];
+    const InstDev& D = table[tile.x];
+    StepCtl& c = *D.ctl;
+    if (!c.pending) return;
+    int ci = 0;
+    while (ci + 1 < D.n_contacts && D.cdesc[ci + 1].node_begin <= tile.y) ++ci;
+    const ContactDesc cd = D.cdesc[ci];
+    const int q = tile.y - cd.node_begin + (int)threadIdx.x;
+    int hit = 0;
+    if (q < cd.n_nodes) {
+        const FaceGrid g = D.cgp[ci];
+        const int32_t* gs = D.cgrid + (size_t)ci * GRID_STRIDE;
+        hit = contact_node(D, cd, q, g, gs, gs + BATCH_GRID_CELLS + 1, step_dt(c));
+    }
+    const unsigned long long m = __ballot(hit != 0 && cd.pair);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(&c.pair_hits, (int)__popcll(m));
+}
+
+// (3) reactions of every contact face; (4) contact force of every touched
+// node.
+template <int T>
+__global__ __launch_bounds__(T) void k_faceforce_tiles(const InstDev* __restrict__ table,
+                                                       const int2* __restrict__ tiles,
+                                                       const int32_t* __restrict__ n_tiles) {
+    if ((int)blockIdx.x >= *n_tiles) return;
+    const int2 tile = tiles[blockIdx.x];
+    const InstDev& D = table[tile.x];
+    if (!D.ctl->pending) return;
+    const int f = tile.y + (int)threadIdx.x;
+    if (f < D.F) contact_face_force(D, f);
+}
+
+template <int T>
+__global__ __launch_bounds__(T) void k_apply_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
+                                                   const int32_t* __restrict__ n_tiles) {
+    if ((int)blockIdx.x >= *n_tiles) return;
+    const int2 tile = tiles[blockIdx.x];
+    const InstDev& D = table[tile.x];
+    if (!D.ctl->pending) return;
+    const int i = tile.y + (int)threadIdx.x;
+    if (i < D.T) contact_apply_node(D, i);
 }
 
 // First half step of every node of the open steps (src/solver.rs:313-323),
@@ -401,7 +447,7 @@ __global__ __launch_bounds__(T) void k_half_tiles(const InstDev* __restrict__ ta
     const int n = tile.y + (int)threadIdx.x;
     if (n >= D.N) return;
     if (c.scale_now) mass_node(D, n);
-    half_node(D, n, step_dt(c));
+    if (!half_node(D, n, step_dt(c))) atomicOr(&D.ctl->bad_step, 1);
 }
 
 // Element forces of the open steps: one workgroup per tile of T element
@@ -452,7 +498,8 @@ __global__ __launch_bounds__(T) void k_gather_tiles(const InstDev* __restrict__ 
     if (!c.pending) return;
     const int n = tile.y + (int)threadIdx.x;
     if (n >= D.N) return;
-    if (!gather_node(D, n, step_dt(c), c.damping)) atomicOr(&c.bad_step, 1);
+    // (the displacement was checked by the half step)
+    if (!gather_node<false>(D, n, step_dt(c), c.damping)) atomicOr(&c.bad_step, 1);
 }
 
 // Compact per-instance summary for the host.
@@ -481,6 +528,11 @@ __global__ void k_stamp(unsigned long long* p) { *p = wall_clock64(); }
 
 // ------------------------------------------------------------- host
 
+// Active lists: the instance list and the tile lists of the tile kernels.
+enum { L_INST, L_HC, L_EL, L_NODE, L_FACE, L_GRID, L_SEARCH, L_APPLY, N_LISTS };
+// Kernels of one step, in launch order.
+enum { K_END, K_GRID, K_SEARCH, K_FACEFORCE, K_APPLY, K_HALF, K_FORCE_HC, K_FORCE_EL, K_GATHER, N_KERNELS };
+
 struct BatchImpl {
     crush_batch_options opt{};
     std::vector<Prepared> prep;
@@ -489,15 +541,15 @@ struct BatchImpl {
     DevBuf<uint8_t> pool;
     DevBuf<InstDev> table;
     DevBuf<double> dtp;  // stable-step partials of every instance (min, then sum)
-    // active lists (instances; honeycomb, elastic and node tiles) and their
-    // counts, device ints read by the kernels
+    // active lists: the instances (d_list) and the tiles of each tile
+    // kernel (d_tiles[L_*]); their counts are device ints (d_counts[L_*])
+    // read by the kernels
     DevBuf<int32_t> d_list, d_counts, n_done;
-    DevBuf<int2> d_tiles_hc, d_tiles_el, d_tiles_node;
+    DevBuf<int2> d_tiles[N_LISTS];
     DevBuf<uint8_t> done_flag;
     std::vector<uint8_t> host_done;
-    int listed = 0;  // instances in the current active list
-    int n_hc = 0, n_el = 0, n_node = 0;  // tiles in the current lists
-    int cap_inst = 0, cap_hc = 0, cap_el = 0, cap_node = 0;  // grids of the captured graph
+    int cnt[N_LISTS] = {};  // entries in the current lists
+    int cap[N_LISTS] = {};  // grids of the captured graph
     bool fresh = true;  // no step taken since creation or reset
     DevBuf<BatchSummary> summary;
     std::vector<BatchSummary> host_summary;
@@ -510,11 +562,10 @@ struct BatchImpl {
     double prepare_ms = 0, upload_ms = 0;
     long long launches = 0, replays = 0;
     // kernel timing: with it on, each replay launches its steps directly,
-    // with events ev[6r..6r+5] before each of the five kernels of step r
-    // and after the last
+    // with an event before each kernel of a step and one after the last
     bool time_kernels = false;
     std::vector<hipEvent_t> ev;
-    double t_inst = 0, t_half = 0, t_hc = 0, t_el = 0, t_gather = 0;
+    double t_kernel[N_KERNELS] = {};
 
     ~BatchImpl() {
         if (graph) (void)hipGraphExecDestroy(graph);
@@ -524,7 +575,7 @@ struct BatchImpl {
 };
 
 constexpr int NODE_TILE = 256;
-constexpr int EV_PER_STEP = 6;
+constexpr int CONTACT_TILE = 64;
 
 static int inst_block(const crush_batch_options& o) {
     const int b = o.threads_per_instance;
@@ -541,68 +592,100 @@ static int steps_per_graph(const crush_batch_options& o) { return o.steps_per_gr
 // Stable-step partial slots of one instance (enough for the smallest tile).
 static size_t dtp_slots(const Prepared& P) { return (size_t)(P.n_hc + 63) / 64 + (P.n_elems - P.n_hc + 63) / 64 + 2; }
 
+// Tiles of instance i for list L with force tile T.
+static void add_tiles(const Prepared& P, int i, int T, std::vector<int2>* out) {
+    for (int s = 0; s < P.n_hc; s += T) out[L_HC].push_back(make_int2(i, s));
+    for (int s = P.n_hc; s < P.n_elems; s += T) out[L_EL].push_back(make_int2(i, s));
+    for (int k = 0; k < P.n_nodes; k += NODE_TILE) out[L_NODE].push_back(make_int2(i, k));
+    for (int f = 0; f < P.n_cfaces; f += CONTACT_TILE) out[L_FACE].push_back(make_int2(i, f));
+    for (int ci = 0; ci < (int)P.cdesc.size(); ++ci) {
+        const ContactDesc& cd = P.cdesc[ci];
+        if (cd.n_faces == 0) continue;
+        out[L_GRID].push_back(make_int2(i, ci));
+        for (int q = 0; q < cd.n_nodes; q += CONTACT_TILE) out[L_SEARCH].push_back(make_int2(i, cd.node_begin + q));
+    }
+    for (int k = 0; k < (int)P.tn_node.size(); k += CONTACT_TILE) out[L_APPLY].push_back(make_int2(i, k));
+}
+
 // Rebuild the active lists from the instances not yet retired.
 static void rebuild_lists(BatchImpl& bi) {
     const int n = (int)bi.prep.size(), T = force_tile(bi.opt);
     std::vector<int32_t> list;
-    std::vector<int2> hc, el, nd;
+    std::vector<int2> tiles[N_LISTS];
     for (int i = 0; i < n; ++i) {
         if (bi.host_done[i]) continue;
         list.push_back(i);
-        const Prepared& P = bi.prep[i];
-        for (int s = 0; s < P.n_hc; s += T) hc.push_back(make_int2(i, s));
-        for (int s = P.n_hc; s < P.n_elems; s += T) el.push_back(make_int2(i, s));
-        for (int k = 0; k < P.n_nodes; k += NODE_TILE) nd.push_back(make_int2(i, k));
+        add_tiles(bi.prep[i], i, T, tiles);
     }
-    auto put = [&](void* dst, const void* src, size_t bytes) {
-        if (bytes) HIP_OK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, bi.stream));
-    };
-    put(bi.d_list.p, list.data(), list.size() * sizeof(int32_t));
-    put(bi.d_tiles_hc.p, hc.data(), hc.size() * sizeof(int2));
-    put(bi.d_tiles_el.p, el.data(), el.size() * sizeof(int2));
-    put(bi.d_tiles_node.p, nd.data(), nd.size() * sizeof(int2));
-    const int32_t counts[4] = {(int32_t)list.size(), (int32_t)hc.size(), (int32_t)el.size(), (int32_t)nd.size()};
-    put(bi.d_counts.p, counts, sizeof counts);
+    int32_t counts[N_LISTS];
+    counts[L_INST] = (int32_t)list.size();
+    if (!list.empty())
+        HIP_OK(hipMemcpyAsync(bi.d_list.p, list.data(), list.size() * sizeof(int32_t), hipMemcpyHostToDevice,
+                              bi.stream));
+    for (int L = 1; L < N_LISTS; ++L) {
+        counts[L] = (int32_t)tiles[L].size();
+        if (!tiles[L].empty())
+            HIP_OK(hipMemcpyAsync(bi.d_tiles[L].p, tiles[L].data(), tiles[L].size() * sizeof(int2),
+                                  hipMemcpyHostToDevice, bi.stream));
+    }
+    HIP_OK(hipMemcpyAsync(bi.d_counts.p, counts, sizeof counts, hipMemcpyHostToDevice, bi.stream));
     HIP_OK(hipStreamSynchronize(bi.stream));
-    bi.listed = counts[0];
-    bi.n_hc = counts[1];
-    bi.n_el = counts[2];
-    bi.n_node = counts[3];
+    for (int L = 0; L < N_LISTS; ++L) bi.cnt[L] = counts[L];
 }
 
 template <int B, int T>
 static void launch_step(BatchImpl& bi, double cs, hipEvent_t* ev) {
     const int32_t* cnt = bi.d_counts.p;
-    if (ev) HIP_OK(hipEventRecord(ev[0], bi.stream));
-    hipLaunchKernelGGL(k_inst<B>, dim3(std::max(bi.cap_inst, 1)), dim3(B), 0, bi.stream, bi.table.p, bi.d_list.p, cnt,
-                       cs, T, bi.done_flag.p, bi.n_done.p);
-    if (ev) HIP_OK(hipEventRecord(ev[1], bi.stream));
-    if (bi.cap_node > 0)
-        hipLaunchKernelGGL(k_half_tiles<NODE_TILE>, dim3(bi.cap_node), dim3(NODE_TILE), 0, bi.stream, bi.table.p,
-                           bi.d_tiles_node.p, cnt + 3);
-    if (ev) HIP_OK(hipEventRecord(ev[2], bi.stream));
-    if (bi.cap_hc > 0)
-        hipLaunchKernelGGL((k_force_tiles<CRUSH_MAT_HONEYCOMB, T>), dim3(bi.cap_hc), dim3(T), 0, bi.stream, bi.table.p,
-                           bi.d_tiles_hc.p, cnt + 1);
-    if (ev) HIP_OK(hipEventRecord(ev[3], bi.stream));
-    if (bi.cap_el > 0)
-        hipLaunchKernelGGL((k_force_tiles<CRUSH_MAT_ELASTIC, T>), dim3(bi.cap_el), dim3(T), 0, bi.stream, bi.table.p,
-                           bi.d_tiles_el.p, cnt + 2);
-    if (ev) HIP_OK(hipEventRecord(ev[4], bi.stream));
-    if (bi.cap_node > 0)
-        hipLaunchKernelGGL(k_gather_tiles<NODE_TILE>, dim3(bi.cap_node), dim3(NODE_TILE), 0, bi.stream, bi.table.p,
-                           bi.d_tiles_node.p, cnt + 3);
-    if (ev) HIP_OK(hipEventRecord(ev[5], bi.stream));
+    const InstDev* tab = bi.table.p;
+    hipStream_t st = bi.stream;
+    auto mark = [&](int k) {
+        if (ev) HIP_OK(hipEventRecord(ev[k], st));
+    };
+    mark(K_END);
+    hipLaunchKernelGGL(k_end<B>, dim3(std::max(bi.cap[L_INST], 1)), dim3(B), 0, st, tab, bi.d_list.p, cnt + L_INST,
+                       T, bi.done_flag.p, bi.n_done.p);
+    mark(K_GRID);
+    if (bi.cap[L_GRID] > 0)
+        hipLaunchKernelGGL(k_grid<B>, dim3(bi.cap[L_GRID]), dim3(B), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID, cs);
+    mark(K_SEARCH);
+    if (bi.cap[L_SEARCH] > 0)
+        hipLaunchKernelGGL(k_search_tiles<CONTACT_TILE>, dim3(bi.cap[L_SEARCH]), dim3(CONTACT_TILE), 0, st, tab,
+                           bi.d_tiles[L_SEARCH].p, cnt + L_SEARCH);
+    mark(K_FACEFORCE);
+    if (bi.cap[L_FACE] > 0)
+        hipLaunchKernelGGL(k_faceforce_tiles<CONTACT_TILE>, dim3(bi.cap[L_FACE]), dim3(CONTACT_TILE), 0, st, tab,
+                           bi.d_tiles[L_FACE].p, cnt + L_FACE);
+    mark(K_APPLY);
+    if (bi.cap[L_APPLY] > 0)
+        hipLaunchKernelGGL(k_apply_tiles<CONTACT_TILE>, dim3(bi.cap[L_APPLY]), dim3(CONTACT_TILE), 0, st, tab,
+                           bi.d_tiles[L_APPLY].p, cnt + L_APPLY);
+    mark(K_HALF);
+    if (bi.cap[L_NODE] > 0)
+        hipLaunchKernelGGL(k_half_tiles<NODE_TILE>, dim3(bi.cap[L_NODE]), dim3(NODE_TILE), 0, st, tab,
+                           bi.d_tiles[L_NODE].p, cnt + L_NODE);
+    mark(K_FORCE_HC);
+    if (bi.cap[L_HC] > 0)
+        hipLaunchKernelGGL((k_force_tiles<CRUSH_MAT_HONEYCOMB, T>), dim3(bi.cap[L_HC]), dim3(T), 0, st, tab,
+                           bi.d_tiles[L_HC].p, cnt + L_HC);
+    mark(K_FORCE_EL);
+    if (bi.cap[L_EL] > 0)
+        hipLaunchKernelGGL((k_force_tiles<CRUSH_MAT_ELASTIC, T>), dim3(bi.cap[L_EL]), dim3(T), 0, st, tab,
+                           bi.d_tiles[L_EL].p, cnt + L_EL);
+    mark(K_GATHER);
+    if (bi.cap[L_NODE] > 0)
+        hipLaunchKernelGGL(k_gather_tiles<NODE_TILE>, dim3(bi.cap[L_NODE]), dim3(NODE_TILE), 0, st, tab,
+                           bi.d_tiles[L_NODE].p, cnt + L_NODE);
+    mark(N_KERNELS);
 }
 
 template <int T>
 static void launch_dt_init(BatchImpl& bi) {
-    if (bi.n_hc > 0)
-        hipLaunchKernelGGL((k_dt_tiles<CRUSH_MAT_HONEYCOMB, T>), dim3(bi.n_hc), dim3(T), 0, bi.stream, bi.table.p,
-                           bi.d_tiles_hc.p, bi.d_counts.p + 1);
-    if (bi.n_el > 0)
-        hipLaunchKernelGGL((k_dt_tiles<CRUSH_MAT_ELASTIC, T>), dim3(bi.n_el), dim3(T), 0, bi.stream, bi.table.p,
-                           bi.d_tiles_el.p, bi.d_counts.p + 2);
+    if (bi.cnt[L_HC] > 0)
+        hipLaunchKernelGGL((k_dt_tiles<CRUSH_MAT_HONEYCOMB, T>), dim3(bi.cnt[L_HC]), dim3(T), 0, bi.stream, bi.table.p,
+                           bi.d_tiles[L_HC].p, bi.d_counts.p + L_HC);
+    if (bi.cnt[L_EL] > 0)
+        hipLaunchKernelGGL((k_dt_tiles<CRUSH_MAT_ELASTIC, T>), dim3(bi.cnt[L_EL]), dim3(T), 0, bi.stream, bi.table.p,
+                           bi.d_tiles[L_EL].p, bi.d_counts.p + L_EL);
 }
 
 template <int B>
@@ -615,6 +698,8 @@ static void launch_step_t(BatchImpl& bi, double cs, hipEvent_t* ev) {
     else
         launch_step<B, 256>(bi, cs, ev);
 }
+
+constexpr int EV_PER_STEP = N_KERNELS + 1;
 
 // One step's launches; ev (EV_PER_STEP events, or null) brackets the kernels.
 static void launch_one(BatchImpl& bi, hipEvent_t* ev) {
@@ -633,12 +718,11 @@ static void launch_one(BatchImpl& bi, hipEvent_t* ev) {
 // Add the kernel durations of the last replay.
 static void read_kernel_times(BatchImpl& bi) {
     if (!bi.time_kernels) return;
-    double* acc[5] = {&bi.t_inst, &bi.t_half, &bi.t_hc, &bi.t_el, &bi.t_gather};
     for (int r = 0; r < steps_per_graph(bi.opt); ++r)
-        for (int k = 0; k < 5; ++k) {
+        for (int k = 0; k < N_KERNELS; ++k) {
             float ms = 0.0f;
             HIP_OK(hipEventElapsedTime(&ms, bi.ev[EV_PER_STEP * r + k], bi.ev[EV_PER_STEP * r + k + 1]));
-            *acc[k] += ms;
+            bi.t_kernel[k] += ms;
         }
 }
 
@@ -648,10 +732,7 @@ static void capture(BatchImpl& bi) {
         HIP_OK(hipGraphExecDestroy(bi.graph));
         bi.graph = nullptr;
     }
-    bi.cap_inst = bi.listed;
-    bi.cap_hc = bi.n_hc;
-    bi.cap_el = bi.n_el;
-    bi.cap_node = bi.n_node;
+    for (int L = 0; L < N_LISTS; ++L) bi.cap[L] = bi.cnt[L];
     const int R = steps_per_graph(bi.opt);
     hipGraph_t g;
     HIP_OK(hipStreamBeginCapture(bi.stream, hipStreamCaptureModeThreadLocal));
@@ -715,9 +796,9 @@ static void reset_state(BatchImpl& bi, bool recapture) {
     bi.done_flag.zero(bi.stream);
     std::fill(bi.host_done.begin(), bi.host_done.end(), 0);
     rebuild_lists(bi);
-    if (recapture || bi.cap_inst != bi.listed || bi.cap_hc != bi.n_hc || bi.cap_el != bi.n_el ||
-        bi.cap_node != bi.n_node)
-        capture(bi);
+    bool same = true;
+    for (int L = 0; L < N_LISTS; ++L) same = same && bi.cap[L] == bi.cnt[L];
+    if (recapture || !same) capture(bi);
     bi.fresh = true;
     bi.summary_valid = false;
 }
@@ -786,18 +867,14 @@ BatchImpl* batch_create(const crush_model* const* models, int n, const crush_bat
                               bi->stream));
     }
     bi->table.upload(tab, bi->stream);
-    // list buffers sized for the smallest tile
-    size_t tiles_hc = 0, tiles_el = 0, tiles_node = 0;
-    for (int i = 0; i < n; ++i) {
-        tiles_hc += (bi->prep[i].n_hc + 63) / 64;
-        tiles_el += (bi->prep[i].n_elems - bi->prep[i].n_hc + 63) / 64;
-        tiles_node += (bi->prep[i].n_nodes + NODE_TILE - 1) / NODE_TILE;
+    // list buffers sized for the smallest force tile
+    {
+        std::vector<int2> tiles[N_LISTS];
+        for (int i = 0; i < n; ++i) add_tiles(bi->prep[i], i, 64, tiles);
+        for (int L = 1; L < N_LISTS; ++L) bi->d_tiles[L].alloc(std::max<size_t>(tiles[L].size(), 1));
     }
     bi->d_list.alloc(n);
-    bi->d_tiles_hc.alloc(std::max<size_t>(tiles_hc, 1));
-    bi->d_tiles_el.alloc(std::max<size_t>(tiles_el, 1));
-    bi->d_tiles_node.alloc(std::max<size_t>(tiles_node, 1));
-    bi->d_counts.alloc(4);
+    bi->d_counts.alloc(N_LISTS);
     bi->n_done.alloc(1);
     bi->n_done.zero(bi->stream);
     bi->done_flag.alloc(n);
@@ -848,11 +925,12 @@ static void drive(BatchImpl& bi, long long max_replays) {
         if (max_replays > 0 && bi.replays >= max_replays) break;
         if (++guard > 100000000LL) fail(CRUSH_E_INPUT, "batch did not finish");
         const int active = n - done;
-        if (active > 0 && 4 * (bi.listed - active) >= bi.listed) {
+        const int listed = bi.cnt[L_INST];
+        if (active > 0 && 4 * (listed - active) >= listed) {
             HIP_OK(hipMemcpyAsync(bi.host_done.data(), bi.done_flag.p, n, hipMemcpyDeviceToHost, bi.stream));
             HIP_OK(hipStreamSynchronize(bi.stream));
             rebuild_lists(bi);
-            if (2 * bi.listed <= bi.cap_inst) capture(bi);
+            if (2 * bi.cnt[L_INST] <= bi.cap[L_INST]) capture(bi);
         }
     }
 }
@@ -867,7 +945,7 @@ void batch_run(BatchImpl& bi, crush_batch_stats* st) {
     hipLaunchKernelGGL(k_stamp, dim3(1), dim3(1), 0, bi.stream, bi.t_start.p);
     bi.launches = 0;
     bi.replays = 0;
-    bi.t_inst = bi.t_half = bi.t_hc = bi.t_el = bi.t_gather = 0.0;
+    for (double& x : bi.t_kernel) x = 0.0;
     drive(bi, 0);
     HIP_OK(hipEventRecord(e1, bi.stream));
     HIP_OK(hipEventSynchronize(e1));
@@ -905,11 +983,13 @@ void batch_run(BatchImpl& bi, crush_batch_stats* st) {
         auto pct = [&](double q) { return lat[std::min<size_t>(n - 1, (size_t)std::ceil(q * n) - 1)]; };
         st->latency_p50_ms = pct(0.50);
         st->latency_p99_ms = pct(0.99);
-        const bool tk = bi.time_kernels;
-        st->k_inst_ms = tk ? bi.t_inst : 0.0;
-        st->k_node_ms = tk ? bi.t_half + bi.t_gather : 0.0;
-        st->k_force_hc_ms = tk ? bi.t_hc : 0.0;
-        st->k_force_el_ms = tk ? bi.t_el : 0.0;
+        const double* k = bi.t_kernel;
+        st->k_end_ms = k[K_END];
+        st->k_contact_ms = k[K_GRID] + k[K_SEARCH] + k[K_FACEFORCE] + k[K_APPLY];
+        st->k_node_ms = k[K_HALF] + k[K_GATHER];
+        st->k_force_hc_ms = k[K_FORCE_HC];
+        st->k_force_el_ms = k[K_FORCE_EL];
+        for (int j = 0; j < N_KERNELS && j < 12; ++j) st->k_ms[j] = k[j];
     }
     for (int i = 0; i < n; ++i)
         if (bi.host_summary[i].error)

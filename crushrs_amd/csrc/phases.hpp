@@ -33,6 +33,8 @@ __device__ __forceinline__ void st_f32(rsrc_t r, float v, unsigned voff, unsigne
 // Everything a step touches for one model (one instance), as device
 // pointers.  The one-solve path fills it from its buffers; the batched path
 // keeps one per instance in a table.
+struct FaceGrid;
+
 struct InstDev {
     int N, E, n_hc, C, F, K, T, n_contacts, n_groups, n_sblk;
     // elements
@@ -94,6 +96,10 @@ struct InstDev {
     // scaling masses), one slot per force tile of the instance
     double* dtp_min;
     double* dtp_sum;
+    // batched path: each contact's face grid in device memory (cell starts
+    // then entries, GRID_STRIDE ints per contact) and its parameters
+    int32_t* cgrid;
+    FaceGrid* cgp;
 };
 
 __device__ __forceinline__ double step_dt(const StepCtl& c) { return fmax(fmin(c.dt_cfl, c.end_time - c.t), 1e-15); }
@@ -227,10 +233,10 @@ __device__ __forceinline__ int contact_node(const InstDev& D, const ContactDesc&
 }
 
 // Sum of the quarter reactions face f receives, in contacting-node order.
-__device__ __forceinline__ void contact_face_force(const InstDev& D, int f) {
+__device__ __forceinline__ void face_reaction(const InstDev& D, int f, double* s) {
 #pragma clang fp contract(off)
     const int cnt = D.fh_count[f];
-    double s[3] = {0.0, 0.0, 0.0};
+    s[0] = s[1] = s[2] = 0.0;
     if (cnt > 0 && cnt <= FACE_HITS_CAP) {
         int ids[FACE_HITS_CAP];
         for (int i = 0; i < cnt; ++i) ids[i] = D.fh_list[(size_t)f * FACE_HITS_CAP + i];
@@ -252,11 +258,19 @@ __device__ __forceinline__ void contact_face_force(const InstDev& D, int f) {
 #pragma unroll
                 for (int d = 0; d < 3; ++d) s[d] += 0.25 * D.rec_f[3 * (size_t)k + d];
     }
+}
+
+__device__ __forceinline__ void contact_face_force(const InstDev& D, int f) {
+    double s[3];
+    face_reaction(D, f, s);
 #pragma unroll
     for (int d = 0; d < 3; ++d) D.face_force[3 * (size_t)f + d] = s[d];
 }
 
-// External (contact) force of touched node i, in contact order.
+// External (contact) force of touched node i, in contact order.  With
+// FUSED the reactions of the node's faces are formed here (the same
+// arithmetic as contact_face_force) instead of read from face_force.
+template <bool FUSED = false>
 __device__ __forceinline__ void contact_apply_node(const InstDev& D, int i) {
 #pragma clang fp contract(off)
     double s[3] = {0.0, 0.0, 0.0};
@@ -268,8 +282,15 @@ __device__ __forceinline__ void contact_apply_node(const InstDev& D, int i) {
                 for (int d = 0; d < 3; ++d) s[d] += D.rec_f[3 * (size_t)e + d];
         } else {
             const int f = -e - 1;
+            double r[3];
+            if (FUSED) {
+                face_reaction(D, f, r);
+            } else {
 #pragma unroll
-            for (int d = 0; d < 3; ++d) s[d] -= D.face_force[3 * (size_t)f + d];
+                for (int d = 0; d < 3; ++d) r[d] = D.face_force[3 * (size_t)f + d];
+            }
+#pragma unroll
+            for (int d = 0; d < 3; ++d) s[d] -= r[d];
         }
     }
     const int n = D.tn_node[i];
@@ -278,7 +299,8 @@ __device__ __forceinline__ void contact_apply_node(const InstDev& D, int i) {
 }
 
 // First half step of node n (src/solver.rs:313-323); v holds v_1/2 after.
-__device__ __forceinline__ void half_node(const InstDev& D, int n, double dt) {
+// Returns false when the new displacement is not finite.
+__device__ __forceinline__ bool half_node(const InstDev& D, int n, double dt) {
 #pragma clang fp contract(off)
     if (D.fixed[n]) {
 #pragma unroll
@@ -286,17 +308,21 @@ __device__ __forceinline__ void half_node(const InstDev& D, int n, double dt) {
             D.u[3 * (size_t)n + d] = 0.0;
             D.v[3 * (size_t)n + d] = 0.0;
         }
-        return;
+        return true;
     }
     const double im = D.inv_mass[n];
+    bool ok = true;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const size_t k = 3 * (size_t)n + d;
         const double a = (D.f_ext[k] - D.f_int[k]) * im;
         const double vh = D.v[k] + 0.5 * dt * a;
         D.v[k] = vh;
-        D.u[k] += dt * vh;
+        const double un = D.u[k] + dt * vh;
+        D.u[k] = un;
+        ok = ok && isfinite(un);
     }
+    return ok;
 }
 
 // Element force of slot s (f32), streamed so that few values are live at
@@ -419,6 +445,7 @@ __device__ __forceinline__ int force_element(const InstDev& D, int s) {
 // Node-centric gather of the element forces (f64 sums in element order)
 // and the second half step (src/solver.rs:327-342).  Returns false when the
 // node's state is not finite.
+template <bool CHECK_U = true>
 __device__ __forceinline__ bool gather_node(const InstDev& D, int n, double dt, double damp) {
 #pragma clang fp contract(off)
     double f[3] = {0.0, 0.0, 0.0};
@@ -440,7 +467,7 @@ __device__ __forceinline__ bool gather_node(const InstDev& D, int n, double dt, 
         double vn = (D.v[k] + 0.5 * dt * a) * damp;
         if (fx) vn = 0.0;
         D.v[k] = vn;
-        ok = ok && isfinite(vn) && isfinite(D.u[k]);
+        ok = ok && isfinite(vn) && (!CHECK_U || isfinite(D.u[k]));
     }
     return ok;
 }

@@ -224,10 +224,17 @@ typedef struct crush_batch_stats {
     double latency_p50_ms;        /* median time from run start to retirement */
     double latency_p99_ms;        /* 99th percentile of the same              */
     int64_t hc_element_steps;     /* honeycomb part of element_steps          */
-    double k_inst_ms;             /* with kernel timing on, summed durations: */
-    double k_node_ms;             /*   per-instance kernel, node kernels      */
-    double k_force_hc_ms;         /*   (half + gather), honeycomb and elastic */
-    double k_force_el_ms;         /*   force kernels                          */
+    /* With kernel timing on (crush_batch_time_kernels), summed durations of
+     * the kernels of the run, in ms: */
+    double k_end_ms;              /* per-instance end-of-step kernel          */
+    double k_contact_ms;          /* contact: faces and grids, search, face
+                                     reactions, apply                        */
+    double k_node_ms;             /* node half step and gather                */
+    double k_force_hc_ms;         /* honeycomb element forces                 */
+    double k_force_el_ms;         /* elastic element forces                   */
+    double k_ms[12];              /* each kernel of a step in launch order:
+                                     end, faces + grid, search, reactions,
+                                     apply, half, honeycomb, elastic, gather */
 } crush_batch_stats;
 
 /* Fill opt with the defaults. */

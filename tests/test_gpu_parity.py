@@ -150,3 +150,22 @@ def test_erosion_decisions_match():
     s = done.value
     assert oe[:s].any(), "the case must erode"
     assert np.array_equal(ge[:s], oe[:s])
+
+
+def test_navigator_neon_head_on_matches_oracle_and_nhtsa_4429():
+    """tests/impact.rs:284-309 on the GPU: delta-v at the rear
+    accelerometers within the reference's NHTSA 4429 bands and the
+    per-vehicle delta-v / crush of the oracle."""
+    print("case: navigator vs neon-pulse 30.14 mph, road, full solve", flush=True)
+    a = ca.renamed(ca.vehicle("navigator"), "navigator_a")
+    b = ca.vehicle("neon-pulse")
+    b.mass = 1378.0
+    b = ca.renamed(b, "neon_b")
+    v0 = 30.14 * ca.MPH
+    m = ca.Model.headon(a, b, v0, v0)
+    g, _ = end_parity(m)
+    v = m.view
+    dv_a = v0 - g.velocity[v.groups[0].accel_node][0]
+    dv_b = v0 + g.velocity[v.groups[1].accel_node][0]
+    assert abs(dv_a - 9.89) < 1.0, dv_a
+    assert abs(dv_b - 19.57) < 2.0, dv_b

@@ -319,3 +319,109 @@ def test_hencky_pressure_and_objectivity():
     lam = e * nu / ((1 + nu) * (1 - 2 * nu)); mu = e / (2 * (1 + nu))
     lin = lam * np.trace(h) * np.eye(3) + 2 * mu * h
     assert np.linalg.norm(t1.reshape(3, 3) - lin) < 1e-4 * np.linalg.norm(lin)
+
+
+def _penetrated_crash(push=0.05):
+    """crash 30 30 with the Neon block pushed `push` m into the Silverado."""
+    import crushrs_amd as ca
+    m = ca.Model.crash(30.0, 30.0)
+    x = m.nodes()
+    u = np.zeros_like(x)
+    neon = m.node_set("neon")
+    u[neon, 0] -= push
+    return m, u
+
+
+def _lumped(m):
+    """Lumped nodal masses (src/model.rs:268-278) from a one-step oracle solve."""
+    import crushrs_amd as ca
+    s = m.settings
+    end = s.end_time
+    s.end_time = 1e-12
+    m.settings = s
+    r = of.run(m)
+    s.end_time = end
+    m.settings = s
+    return r.masses
+
+
+def _contact(m, u, dt=1e-4):
+    f = np.zeros(3 * m.n_nodes)
+    hit = np.full(max(m.contact_node_count(), 1), -2, np.int32)
+    masses = np.ascontiguousarray(_lumped(m))
+    n = of.load().oracle_contact_apply(m.ptr, of.ptr(np.ascontiguousarray(u.reshape(-1))), of.ptr(masses), dt,
+                                       of.ptr(f), of.ptr(hit))
+    return n, f.reshape(-1, 3), hit
+
+
+def test_contact_forces_balance_and_only_penetrating_nodes_push():
+    """src/contact.rs:145-154: each node force +f n is balanced by -f n/4 on
+    the four face nodes, so the contact forces sum to zero; untouched nodes
+    get none."""
+    m, u = _penetrated_crash(0.05)
+    n, f, hit = _contact(m, u)
+    assert n > 0
+    total = f.sum(axis=0)
+    assert np.all(np.abs(total) < 1e-9 * np.abs(f).sum())
+    # force limit m * 1e5 (src/contact.rs:144) and penalty direction
+    assert np.all(np.isfinite(f))
+    assert (hit >= -1).all()
+
+
+def test_contact_first_hit_is_lowest_face_index_brute_force():
+    """src/contact.rs:113-157: for a penetrating node the face that takes
+    it is the first face in order that accepts it.  The oracle's choice is
+    re-derived here by brute force over every face with the same tests
+    (plane distance within [-max_d, 0], projection inside one of the four
+    triangles of either diagonal split, tolerance 0.1)."""
+    m, u = _penetrated_crash(0.05)
+    n, f, hit = _contact(m, u)
+    v = m.view
+    x = m.nodes() + u
+    faces = np.ctypeslib.as_array(v.faces, shape=(v.n_faces, 4))
+    off = 0
+    checked = 0
+    for ci in range(v.n_contacts):
+        c = v.contacts[ci]
+        cnodes = np.ctypeslib.as_array(c.nodes, shape=(c.n_nodes,))
+        cfaces = np.ctypeslib.as_array(c.faces, shape=(c.n_faces,))
+        # nodes kept by the static broad phase (src/contact.rs:48-70)
+        x0 = m.nodes()
+        fx = x0[faces[cfaces].reshape(-1)]
+        lo, hi = fx.min(0) - c.max_distance, fx.max(0) + c.max_distance
+        kept = [nd for nd in cnodes if np.all(x0[nd] >= lo) and np.all(x0[nd] <= hi)]
+        for j, nd in enumerate(kept):
+            want = -1
+            for fi, face in enumerate(cfaces):
+                q = x[faces[face]]
+                nrm = np.cross(q[1] - q[0], q[2] - q[0])
+                nrm /= np.linalg.norm(nrm)
+                ctr = q.mean(0)
+                bl, bh = q.min(0) - c.max_distance, q.max(0) + c.max_distance
+                p = x[nd]
+                if np.any(p < bl) or np.any(p > bh):
+                    continue
+                s = nrm @ (p - ctr)
+                if not (-c.max_distance <= s <= 0.0):
+                    continue
+                pp = p - s * nrm
+                inside = False
+                for a, b, cc in ((0, 1, 2), (0, 2, 3), (1, 2, 3), (1, 3, 0)):
+                    v0, v1, v2 = q[cc] - q[a], q[b] - q[a], pp - q[a]
+                    d00, d01, d11 = v0 @ v0, v0 @ v1, v1 @ v1
+                    d20, d21 = v2 @ v0, v2 @ v1
+                    den = d00 * d11 - d01 * d01
+                    uu = (d11 * d20 - d01 * d21) / den
+                    vv = (d00 * d21 - d01 * d20) / den
+                    if uu >= -0.1 and vv >= -0.1 and uu + vv <= 1.1:
+                        inside = True
+                        break
+                if inside:
+                    want = fi
+                    break
+            got = hit[off + j]
+            if want >= 0 or got >= 0:
+                checked += 1
+            assert got == want, (ci, j, nd, got, want)
+        off += len(kept)
+    assert checked > 0
