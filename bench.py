@@ -49,7 +49,7 @@ HBM_PEAK_GBS = 8000.0
 B_HC, B_EL, B_NODE_FORCE, B_NODE_UPDATE = 776, 576, 36, 76
 B_DT_ELEM, B_DT_NODE = 36, 24
 KERNELS = ("k_end", "k_grid", "k_search_tiles", "k_faceforce_tiles", "k_apply_tiles", "k_half_tiles",
-           "k_force_tiles<honeycomb>", "k_force_tiles<elastic>", "k_gather_tiles")
+           "k_force_hc", "k_force_el", "k_gather_tiles")
 
 
 def step_bytes(n_hc: int, n_el: int, n_nodes: int, steps: int, check_every: int = 5) -> float:
@@ -130,6 +130,32 @@ def plain_gpu(models, k: int = 8):
     tot = sum(t)
     return {"value": len(t) / tot, "unit": "solves/s", "sample": f"{len(t)} sampled models solved one at a time",
             "gelem_steps_per_s": es / tot / 1e9, "ms_per_solve_p50": float(np.median(t) * 1e3)}
+
+
+def copy_probe(nbytes: int) -> None:
+    """One device-to-device copy of nbytes with the HIP runtime's copy
+    kernel: the reference reading for the HBM byte counters."""
+    import ctypes as C
+    ca._abi.load()  # loads the HIP runtime the library links
+    hip = None
+    for name in ("libamdhip64.so", "libamdhip64.so.7", "libamdhip64.so.6", "/opt/rocm/lib/libamdhip64.so"):
+        try:
+            hip = C.CDLL(name)
+            break
+        except OSError:
+            continue
+    if hip is None:
+        raise RuntimeError("HIP runtime not found")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 1, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    rc = hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3)  # hipMemcpyDeviceToDevice
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"copy_probe_bytes": nbytes, "rc": rc}))
 
 
 def bench_sweep(args):
@@ -215,7 +241,7 @@ def bench_sweep(args):
                  "pool_gib": st.pool_bytes / 2**30},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_force_tiles<honeycomb>", "launch_ms": avg_ms,
+                     "kernel": "k_force_hc", "launch_ms": avg_ms,
                      "algorithmic_bytes_per_launch": bytes_per_launch, "bytes_per_element_step": B_HC},
         "kernels": kernels,
     }
@@ -298,13 +324,7 @@ def main():
         print(json.dumps({"error": "no HIP device"}))
         sys.exit(1)
     if args.copy_probe:
-        import torch
-        a = torch.ones(args.copy_probe // 4, dtype=torch.float32, device="cuda")
-        b = torch.empty_like(a)
-        torch.cuda.synchronize()
-        b.copy_(a)
-        torch.cuda.synchronize()
-        print(json.dumps({"copy_probe_bytes": args.copy_probe}))
+        copy_probe(args.copy_probe)
         return
     if args.workload == "sweep":
         args.steps = 1 if args.steps is None else args.steps

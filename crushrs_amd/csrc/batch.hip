@@ -2,17 +2,21 @@
 //
 // Each instance is one contiguous image in a device pool (its arrays plus a
 // control block) described by an InstDev pointer table.  One explicit step
-// of every active instance is five launches:
-//   k_inst          one workgroup per instance: end of the previous step,
-//                   stable step (at check steps) and contact;
-//   k_half_tiles    first half step, one workgroup per tile of 256 nodes of
-//                   one instance, over all instances at once;
-//   k_force_tiles   element forces (and the next check step's stable-step
-//                   partials), one workgroup per tile of element slots of
-//                   one instance, honeycomb and elastic tiles in separate
-//                   launches so each fills the GPU at its own register
-//                   budget;
-//   k_gather_tiles  node-centric gather and second half step, node tiles.
+// of every active instance is nine launches, each over all instances:
+//   k_end              one workgroup per instance: end of the previous step
+//                      and the stable step of the next (at check steps);
+//   k_grid             one workgroup per contact of an instance: face
+//                      geometry and the face grid, stored to device memory;
+//   k_search_tiles     node search against the grid (tiles of contacting
+//                      nodes); k_faceforce_tiles face reactions (tiles of
+//                      faces); k_apply_tiles contact forces (tiles of
+//                      touched nodes);
+//   k_half_tiles       first half step (tiles of 256 nodes);
+//   k_force_hc,        element forces and the next check step's stable-step
+//   k_force_el         partials (tiles of element slots), honeycomb and
+//                      elastic tiles in separate launches so each fills the
+//                      GPU at its own register budget;
+//   k_gather_tiles     node-centric gather and second half step.
 // No arithmetic of an instance depends on the others, so its results are
 // bitwise the same alone or in any mix.  A fixed number of steps is
 // captured in one hipGraph and replayed until every instance has retired;
@@ -455,9 +459,8 @@ __global__ __launch_bounds__(T) void k_half_tiles(const InstDev* __restrict__ ta
 // check step, the tile's stable-step partial is formed at the same
 // configuration (the positions and erosion the next step starts from).
 template <int KIND, int T>
-__global__ __launch_bounds__(T) void k_force_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
-                                                   const int32_t* __restrict__ n_tiles) {
-    __shared__ double s_w[T / 64];
+__device__ __forceinline__ void force_tile(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
+                                           const int32_t* __restrict__ n_tiles, double* s_w) {
     if ((int)blockIdx.x >= *n_tiles) return;
     const int2 tile = tiles[blockIdx.x];
     const InstDev& D = table[tile.x];
@@ -470,6 +473,26 @@ __global__ __launch_bounds__(T) void k_force_tiles(const InstDev* __restrict__ t
     const unsigned long long m = __ballot(er != 0);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(&c.eroded_step, (int)__popcll(m));
     if (c.adaptive && (c.step + 1) % c.chk == 0) dt_partial<T>(D, c.dt_floor, s, end, dt_slot<KIND, T>(D, tile.y), s_w);
+}
+
+// Waves per SIMD the honeycomb kernel is compiled for: 2 (194 VGPRs, no
+// spills); 3 forces spills that cost more than the extra wave gains.
+#ifndef CRUSH_HC_WAVES
+#define CRUSH_HC_WAVES 2
+#endif
+
+template <int T>
+__global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(CRUSH_HC_WAVES, 8))) void k_force_hc(
+    const InstDev* __restrict__ table, const int2* __restrict__ tiles, const int32_t* __restrict__ n_tiles) {
+    __shared__ double s_w[T / 64];
+    force_tile<CRUSH_MAT_HONEYCOMB, T>(table, tiles, n_tiles, s_w);
+}
+
+template <int T>
+__global__ __launch_bounds__(T) void k_force_el(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
+                                                const int32_t* __restrict__ n_tiles) {
+    __shared__ double s_w[T / 64];
+    force_tile<CRUSH_MAT_ELASTIC, T>(table, tiles, n_tiles, s_w);
 }
 
 // Stable-step partials of instances that have not taken a step yet.
@@ -665,11 +688,11 @@ static void launch_step(BatchImpl& bi, double cs, hipEvent_t* ev) {
                            bi.d_tiles[L_NODE].p, cnt + L_NODE);
     mark(K_FORCE_HC);
     if (bi.cap[L_HC] > 0)
-        hipLaunchKernelGGL((k_force_tiles<CRUSH_MAT_HONEYCOMB, T>), dim3(bi.cap[L_HC]), dim3(T), 0, st, tab,
+        hipLaunchKernelGGL(k_force_hc<T>, dim3(bi.cap[L_HC]), dim3(T), 0, st, tab,
                            bi.d_tiles[L_HC].p, cnt + L_HC);
     mark(K_FORCE_EL);
     if (bi.cap[L_EL] > 0)
-        hipLaunchKernelGGL((k_force_tiles<CRUSH_MAT_ELASTIC, T>), dim3(bi.cap[L_EL]), dim3(T), 0, st, tab,
+        hipLaunchKernelGGL(k_force_el<T>, dim3(bi.cap[L_EL]), dim3(T), 0, st, tab,
                            bi.d_tiles[L_EL].p, cnt + L_EL);
     mark(K_GATHER);
     if (bi.cap[L_NODE] > 0)

@@ -348,6 +348,14 @@ __device__ __forceinline__ int force_element(const InstDev& D, int s) {
     const rsrc_t rhs = make_rsrc(D.chs, (size_t)HSYM * C * 4);
     const int c = D.ecls[s];
     const unsigned vc = (unsigned)c * 4u;
+    // the honeycomb state is loaded first so that its latency overlaps the
+    // gradient pass
+    const rsrc_t rst = make_rsrc(D.state, (size_t)25 * nh * 4);
+    float st[KIND == CRUSH_MAT_HONEYCOMB ? 25 : 1];
+    if (KIND == CRUSH_MAT_HONEYCOMB) {
+#pragma unroll
+        for (int k = 0; k < 25; ++k) st[k] = ld_f32(rst, vs, k * nh * 4u);
+    }
     const unsigned vn0 = (unsigned)ld_i32(rconn, vs, 0) * 24u;
     const double u0x = ld_f64(ru, vn0, 0), u0y = ld_f64(ru, vn0 + 8u, 0), u0z = ld_f64(ru, vn0 + 16u, 0);
     float hd[9], q[12];
@@ -404,10 +412,6 @@ __device__ __forceinline__ int force_element(const InstDev& D, int s) {
     const MatParams& mp = D.mats[D.cls_mat[c]];
     float tau[9];
     if (KIND == CRUSH_MAT_HONEYCOMB) {
-        const rsrc_t rst = make_rsrc(D.state, (size_t)25 * nh * 4);
-        float st[25];
-#pragma unroll
-        for (int k = 0; k < 25; ++k) st[k] = ld_f32(rst, vs, k * nh * 4u);
         honeycomb_update(mp, hd, st, tau);
 #pragma unroll
         for (int k = 0; k < 25; ++k) st_f32(rst, st[k], vs, k * nh * 4u);
@@ -450,9 +454,28 @@ __device__ __forceinline__ bool gather_node(const InstDev& D, int n, double dt, 
 #pragma clang fp contract(off)
     double f[3] = {0.0, 0.0, 0.0};
     const int E = D.E;
-    for (int k = D.n2e_ptr[n]; k < D.n2e_ptr[n + 1]; ++k) {
-        const int code = D.n2e_idx[k];
-        const size_t base = (size_t)(3 * (code & 7)) * E + (code >> 3);
+    const int kb = D.n2e_ptr[n], ke = D.n2e_ptr[n + 1];
+    // up to 8 incidences (every node of a hex mesh without merged nodes):
+    // all codes, then all element forces, are loaded before the sums so
+    // that the loads overlap; the sums keep element order
+    float g[8][3];
+    int code[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) code[j] = kb + j < ke ? D.n2e_idx[kb + j] : -1;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const size_t base = code[j] >= 0 ? (size_t)(3 * (code[j] & 7)) * E + (code[j] >> 3) : 0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) g[j][d] = code[j] >= 0 ? D.fe[base + (size_t)d * E] : 0.0f;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+        if (code[j] >= 0)
+#pragma unroll
+            for (int d = 0; d < 3; ++d) f[d] += (double)g[j][d];
+    for (int k = kb + 8; k < ke; ++k) {
+        const int cd = D.n2e_idx[k];
+        const size_t base = (size_t)(3 * (cd & 7)) * E + (cd >> 3);
 #pragma unroll
         for (int d = 0; d < 3; ++d) f[d] += (double)D.fe[base + (size_t)d * E];
     }
