@@ -325,9 +325,81 @@ class Batch:
         return to_results(r, u, v, m, g)
 
 
+def server_options(**kw) -> _abi.ServerOptions:
+    """crush_server_defaults with fields overridden by keyword."""
+    o = _abi.ServerOptions()
+    load().crush_server_defaults(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    return o
+
+
+class Server:
+    """Continuous-batching request loop on the GPU (crush_server_*): submit
+    models from any thread, poll or wait for results."""
+
+    def __init__(self, options: Optional[_abi.ServerOptions] = None, **kw):
+        self._lib = load()
+        opt = options if options is not None else server_options(**kw)
+        self._h = C.c_void_p()
+        check(self._lib.crush_server_create(C.byref(opt), C.byref(self._h)))
+        self._shape = {}
+
+    def close(self) -> None:
+        h = getattr(self, "_h", None)
+        if h:
+            self._lib.crush_server_destroy(h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, model: Model, fields: bool = False) -> int:
+        t = C.c_int64()
+        check(self._lib.crush_server_submit(self._h, model.ptr, _abi.SUBMIT_FIELDS if fields else 0, C.byref(t)))
+        self._shape[t.value] = (model.n_nodes, model.view.n_groups, fields)
+        return t.value
+
+    def _alloc(self, ticket):
+        # an unknown ticket gets empty outputs; the library reports the error
+        n_nodes, n_groups, fields = self._shape.get(ticket, (0, 0, False))
+        r, u, v, m, g = alloc_results(n_nodes, n_groups)
+        if not fields:
+            r.displacement = None
+            r.velocity = None
+        return r, u, v, m, g
+
+    def poll(self, ticket: int) -> Optional[Results]:
+        r, u, v, m, g = self._alloc(ticket)
+        done = C.c_int32()
+        check(self._lib.crush_server_poll(self._h, ticket, C.byref(r), C.byref(done)))
+        if not done.value:
+            return None
+        self._shape.pop(ticket, None)
+        return to_results(r, u, v, m, g)
+
+    def wait(self, ticket: int, timeout_ms: float = 0.0) -> Results:
+        r, u, v, m, g = self._alloc(ticket)
+        check(self._lib.crush_server_wait(self._h, ticket, C.byref(r), timeout_ms))
+        self._shape.pop(ticket, None)
+        return to_results(r, u, v, m, g)
+
+    def stats(self) -> _abi.ServerStats:
+        st = _abi.ServerStats()
+        check(self._lib.crush_server_stats_get(self._h, C.byref(st)))
+        return st
+
+
 def device_count() -> int:
     return int(load().crush_device_count())
 
 
 __all__ = ["MPH", "Model", "Results", "run", "vehicle", "vehicle_file", "side_profile", "rear_profile",
-           "default_settings", "debug_run_steps", "device_count", "CrushError", "renamed", "Batch", "batch_options"]
+           "default_settings", "debug_run_steps", "device_count", "CrushError", "renamed", "Batch", "batch_options", "Server",
+           "server_options"]

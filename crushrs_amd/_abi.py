@@ -207,6 +207,39 @@ class BatchStats(C.Structure):
     ]
 
 
+class ServerOptions(C.Structure):
+    _fields_ = [
+        ("hbm_budget_bytes", C.c_double),
+        ("max_resident", C.c_int32),
+        ("steps_per_graph", C.c_int32),
+        ("threads_per_instance", C.c_int32),
+        ("force_tile", C.c_int32),
+        ("contact_cell_scale", C.c_double),
+        ("max_queued", C.c_int32),
+        ("reserved0", C.c_int32),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_int64),
+        ("admitted", C.c_int64),
+        ("completed", C.c_int64),
+        ("failed", C.c_int64),
+        ("resident", C.c_int32),
+        ("queued", C.c_int32),
+        ("device_bytes_in_use", C.c_double),
+        ("latency_p50_ms", C.c_double),
+        ("latency_p99_ms", C.c_double),
+        ("queue_p50_ms", C.c_double),
+        ("busy_ms", C.c_double),
+        ("element_steps", C.c_int64),
+        ("replays", C.c_int64),
+    ]
+
+
+SUBMIT_FIELDS = 1
+
 LIB_PATH = Path(__file__).resolve().parent / "libcrushrs_amd.so"
 
 # Every symbol the public headers declare, with (restype, argtypes).
@@ -226,6 +259,13 @@ SIGNATURES = {
     "crush_batch_time_kernels": (C.c_int32, [C.c_void_p, C.c_int32]),
     "crush_batch_result": (C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(Results)]),
     "crush_batch_free": (None, [C.c_void_p]),
+    "crush_server_defaults": (None, [C.POINTER(ServerOptions)]),
+    "crush_server_create": (C.c_int32, [C.POINTER(ServerOptions), C.POINTER(C.c_void_p)]),
+    "crush_server_submit": (C.c_int32, [C.c_void_p, C.POINTER(ModelView), C.c_int32, C.POINTER(C.c_int64)]),
+    "crush_server_poll": (C.c_int32, [C.c_void_p, C.c_int64, C.POINTER(Results), C.POINTER(C.c_int32)]),
+    "crush_server_wait": (C.c_int32, [C.c_void_p, C.c_int64, C.POINTER(Results), C.c_double]),
+    "crush_server_stats_get": (C.c_int32, [C.c_void_p, C.POINTER(ServerStats)]),
+    "crush_server_destroy": (None, [C.c_void_p]),
     "crush_headon_defaults": (None, [C.POINTER(HeadOnOptions)]),
     "crush_vehicle_load": (C.c_int32, [C.c_char_p, C.POINTER(Vehicle)]),
     "crush_vehicle_side_profile": (C.c_int32, [C.POINTER(Vehicle), C.POINTER(Vehicle)]),

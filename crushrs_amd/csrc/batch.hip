@@ -21,7 +21,7 @@
 // bitwise the same alone or in any mix.  A fixed number of steps is
 // captured in one hipGraph and replayed until every instance has retired;
 // retired instances leave the active lists between replays.
-#include "batch.hpp"
+#include "engine.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -38,9 +38,6 @@
 
 namespace crush {
 
-constexpr int BATCH_GRID_CELLS = 1024;
-constexpr int BATCH_GRID_ENTRIES = 4096;
-constexpr int GRID_STRIDE = BATCH_GRID_CELLS + 1 + BATCH_GRID_ENTRIES;
 
 // ------------------------------------------------------------- images
 
@@ -68,18 +65,6 @@ T* at(uintptr_t base, size_t off) {
 
 }  // namespace
 
-// Offsets of every array of an instance image relative to its start; the
-// first `init_bytes` must be uploaded, the rest is scratch.
-struct ImageOffsets {
-    size_t conn, ecls, cls_mat, cgrad, cvol, cgam, chs, cdtf, emass, mats, state, eroded, extra;
-    size_t X, u, v, f_int, f_ext, inv_mass, mass, added, fixed, n2e_ptr, n2e_idx;
-    size_t cdesc, cn_node, cn_scale, cf_nodes, cf_contact, tn_node, tn_ptr, tn_ent, rec_face, fh_count;
-    size_t g_nodes_ptr, g_nodes, g_front_ptr, g_front, g_accel, g_axis, g_mass, g_v0, g_dv, ctl;
-    size_t init_bytes;
-    size_t fe, fgeo, rec_f, fh_list, face_force, cgrid, cgp;
-    size_t total;
-};
-
 static StepCtl initial_ctl(const Prepared& P) {
     StepCtl c;
     std::memset(&c, 0, sizeof c);
@@ -96,8 +81,11 @@ static StepCtl initial_ctl(const Prepared& P) {
     return c;
 }
 
+// Stable-step partial slots of one instance (enough for the smallest tile).
+static size_t dtp_slots(const Prepared& P) { return (size_t)(P.n_hc + 63) / 64 + (P.n_elems - P.n_hc + 63) / 64 + 2; }
+
 // Host image of one instance (initialized part only).
-static ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
+ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     ImageBuilder b;
     ImageOffsets o{};
     const int N = P.n_nodes, E = P.n_elems;
@@ -173,12 +161,14 @@ static ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     o.face_force = scratch(sizeof(double) * 3 * (size_t)P.n_cfaces);
     o.cgrid = scratch(sizeof(int32_t) * GRID_STRIDE * P.cdesc.size());
     o.cgp = scratch(sizeof(FaceGrid) * P.cdesc.size());
+    o.dtp_min = scratch(sizeof(double) * dtp_slots(P));
+    o.dtp_sum = scratch(sizeof(double) * dtp_slots(P));
     o.total = off;
     out.swap(b.bytes);
     return o;
 }
 
-static InstDev make_inst(const Prepared& P, const ImageOffsets& o, uintptr_t base) {
+InstDev make_inst(const Prepared& P, const ImageOffsets& o, uintptr_t base) {
     InstDev I{};
     I.N = P.n_nodes;
     I.E = P.n_elems;
@@ -243,6 +233,8 @@ static InstDev make_inst(const Prepared& P, const ImageOffsets& o, uintptr_t bas
     I.ctl = at<StepCtl>(base, o.ctl);
     I.cgrid = at<int32_t>(base, o.cgrid);
     I.cgp = at<FaceGrid>(base, o.cgp);
+    I.dtp_min = at<double>(base, o.dtp_min);
+    I.dtp_sum = at<double>(base, o.dtp_sum);
     return I;
 }
 
@@ -551,54 +543,6 @@ __global__ void k_stamp(unsigned long long* p) { *p = wall_clock64(); }
 
 // ------------------------------------------------------------- host
 
-// Active lists: the instance list and the tile lists of the tile kernels.
-enum { L_INST, L_HC, L_EL, L_NODE, L_FACE, L_GRID, L_SEARCH, L_APPLY, N_LISTS };
-// Kernels of one step, in launch order.
-enum { K_END, K_GRID, K_SEARCH, K_FACEFORCE, K_APPLY, K_HALF, K_FORCE_HC, K_FORCE_EL, K_GATHER, N_KERNELS };
-
-struct BatchImpl {
-    crush_batch_options opt{};
-    std::vector<Prepared> prep;
-    std::vector<ImageOffsets> offs;
-    std::vector<size_t> base_off;
-    DevBuf<uint8_t> pool;
-    DevBuf<InstDev> table;
-    DevBuf<double> dtp;  // stable-step partials of every instance (min, then sum)
-    // active lists: the instances (d_list) and the tiles of each tile
-    // kernel (d_tiles[L_*]); their counts are device ints (d_counts[L_*])
-    // read by the kernels
-    DevBuf<int32_t> d_list, d_counts, n_done;
-    DevBuf<int2> d_tiles[N_LISTS];
-    DevBuf<uint8_t> done_flag;
-    std::vector<uint8_t> host_done;
-    int cnt[N_LISTS] = {};  // entries in the current lists
-    int cap[N_LISTS] = {};  // grids of the captured graph
-    bool fresh = true;  // no step taken since creation or reset
-    DevBuf<BatchSummary> summary;
-    std::vector<BatchSummary> host_summary;
-    bool summary_valid = false;
-    DevBuf<unsigned long long> t_start;
-    unsigned long long host_t_start = 0;
-    double clock_khz = 1.0;
-    hipStream_t stream = nullptr;
-    hipGraphExec_t graph = nullptr;
-    double prepare_ms = 0, upload_ms = 0;
-    long long launches = 0, replays = 0;
-    // kernel timing: with it on, each replay launches its steps directly,
-    // with an event before each kernel of a step and one after the last
-    bool time_kernels = false;
-    std::vector<hipEvent_t> ev;
-    double t_kernel[N_KERNELS] = {};
-
-    ~BatchImpl() {
-        if (graph) (void)hipGraphExecDestroy(graph);
-        for (hipEvent_t e : ev) (void)hipEventDestroy(e);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
-
-constexpr int NODE_TILE = 256;
-constexpr int CONTACT_TILE = 64;
 
 static int inst_block(const crush_batch_options& o) {
     const int b = o.threads_per_instance;
@@ -610,10 +554,8 @@ static int force_tile(const crush_batch_options& o) {
     return (t == 64 || t == 128) ? t : 256;
 }
 
-static int steps_per_graph(const crush_batch_options& o) { return o.steps_per_graph > 0 ? o.steps_per_graph : 32; }
+int steps_per_graph(const crush_batch_options& o) { return o.steps_per_graph > 0 ? o.steps_per_graph : 32; }
 
-// Stable-step partial slots of one instance (enough for the smallest tile).
-static size_t dtp_slots(const Prepared& P) { return (size_t)(P.n_hc + 63) / 64 + (P.n_elems - P.n_hc + 63) / 64 + 2; }
 
 // Tiles of instance i for list L with force tile T.
 static void add_tiles(const Prepared& P, int i, int T, std::vector<int2>* out) {
@@ -631,7 +573,7 @@ static void add_tiles(const Prepared& P, int i, int T, std::vector<int2>* out) {
 }
 
 // Rebuild the active lists from the instances not yet retired.
-static void rebuild_lists(BatchImpl& bi) {
+void rebuild_lists(BatchImpl& bi) {
     const int n = (int)bi.prep.size(), T = force_tile(bi.opt);
     std::vector<int32_t> list;
     std::vector<int2> tiles[N_LISTS];
@@ -640,6 +582,16 @@ static void rebuild_lists(BatchImpl& bi) {
         list.push_back(i);
         add_tiles(bi.prep[i], i, T, tiles);
     }
+    // grow the list buffers (a graph captured with the old ones is stale)
+    auto need = [&](auto& buf, size_t want) {
+        if (want > buf.n) {
+            HIP_OK(hipStreamSynchronize(bi.stream));
+            buf.alloc(want + want / 2);
+            bi.need_capture = true;
+        }
+    };
+    need(bi.d_list, list.size());
+    for (int L = 1; L < N_LISTS; ++L) need(bi.d_tiles[L], tiles[L].size());
     int32_t counts[N_LISTS];
     counts[L_INST] = (int32_t)list.size();
     if (!list.empty())
@@ -724,6 +676,25 @@ static void launch_step_t(BatchImpl& bi, double cs, hipEvent_t* ev) {
 
 constexpr int EV_PER_STEP = N_KERNELS + 1;
 
+void dt_init(BatchImpl& bi) {
+    const int T = force_tile(bi.opt);
+    if (T == 64)
+        launch_dt_init<64>(bi);
+    else if (T == 128)
+        launch_dt_init<128>(bi);
+    else
+        launch_dt_init<256>(bi);
+}
+
+void collect(BatchImpl& bi, int n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_collect, dim3((n + 255) / 256), dim3(256), 0, bi.stream, bi.table.p, n, bi.summary.p);
+    bi.host_summary.resize(n);
+    HIP_OK(hipMemcpyAsync(bi.host_summary.data(), bi.summary.p, n * sizeof(BatchSummary), hipMemcpyDeviceToHost,
+                          bi.stream));
+    HIP_OK(hipStreamSynchronize(bi.stream));
+}
+
 // One step's launches; ev (EV_PER_STEP events, or null) brackets the kernels.
 static void launch_one(BatchImpl& bi, hipEvent_t* ev) {
     const double cs = bi.opt.contact_cell_scale > 0.0 ? bi.opt.contact_cell_scale : 1.0;
@@ -750,12 +721,13 @@ static void read_kernel_times(BatchImpl& bi) {
 }
 
 // Capture steps_per_graph steps with grids sized to the current lists.
-static void capture(BatchImpl& bi) {
+void capture(BatchImpl& bi) {
     if (bi.graph) {
         HIP_OK(hipGraphExecDestroy(bi.graph));
         bi.graph = nullptr;
     }
-    for (int L = 0; L < N_LISTS; ++L) bi.cap[L] = bi.cnt[L];
+    for (int L = 0; L < N_LISTS; ++L) bi.cap[L] = (int)((double)bi.cnt[L] * bi.headroom) + (bi.headroom > 1.0 ? 1 : 0);
+    bi.need_capture = false;
     const int R = steps_per_graph(bi.opt);
     hipGraph_t g;
     HIP_OK(hipStreamBeginCapture(bi.stream, hipStreamCaptureModeThreadLocal));
@@ -767,17 +739,10 @@ static void capture(BatchImpl& bi) {
 
 // The launches of one graph replay: the graph, or with kernel timing on,
 // the same steps launched directly with events around each kernel.
-static void replay(BatchImpl& bi) {
+void replay(BatchImpl& bi) {
     const int R = steps_per_graph(bi.opt);
     if (bi.fresh) {
-        // stable-step partials of the first step
-        const int T = force_tile(bi.opt);
-        if (T == 64)
-            launch_dt_init<64>(bi);
-        else if (T == 128)
-            launch_dt_init<128>(bi);
-        else
-            launch_dt_init<256>(bi);
+        dt_init(bi);
         bi.fresh = false;
     }
     if (!bi.time_kernels) {
@@ -819,7 +784,7 @@ static void reset_state(BatchImpl& bi, bool recapture) {
     bi.done_flag.zero(bi.stream);
     std::fill(bi.host_done.begin(), bi.host_done.end(), 0);
     rebuild_lists(bi);
-    bool same = true;
+    bool same = !bi.need_capture;
     for (int L = 0; L < N_LISTS; ++L) same = same && bi.cap[L] == bi.cnt[L];
     if (recapture || !same) capture(bi);
     bi.fresh = true;
@@ -869,34 +834,21 @@ BatchImpl* batch_create(const crush_model* const* models, int n, const crush_bat
     bi->prepare_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     // pool layout
     bi->base_off.resize(n);
-    size_t total = 0, slots = 0;
-    std::vector<size_t> dtp_off(n);
+    size_t total = 0;
     for (int i = 0; i < n; ++i) {
         bi->base_off[i] = total;
         total += (bi->offs[i].total + 255) & ~size_t(255);
-        dtp_off[i] = slots;
-        slots += dtp_slots(bi->prep[i]);
     }
     HIP_OK(hipStreamCreateWithFlags(&bi->stream, hipStreamNonBlocking));
     bi->pool.alloc(total);
-    bi->dtp.alloc(2 * slots);
     std::vector<InstDev> tab(n);
     const uintptr_t base = reinterpret_cast<uintptr_t>(bi->pool.p);
     for (int i = 0; i < n; ++i) {
         tab[i] = make_inst(bi->prep[i], bi->offs[i], base + bi->base_off[i]);
-        tab[i].dtp_min = bi->dtp.p + dtp_off[i];
-        tab[i].dtp_sum = bi->dtp.p + slots + dtp_off[i];
         HIP_OK(hipMemcpyAsync(bi->pool.p + bi->base_off[i], images[i].data(), images[i].size(), hipMemcpyHostToDevice,
                               bi->stream));
     }
     bi->table.upload(tab, bi->stream);
-    // list buffers sized for the smallest force tile
-    {
-        std::vector<int2> tiles[N_LISTS];
-        for (int i = 0; i < n; ++i) add_tiles(bi->prep[i], i, 64, tiles);
-        for (int L = 1; L < N_LISTS; ++L) bi->d_tiles[L].alloc(std::max<size_t>(tiles[L].size(), 1));
-    }
-    bi->d_list.alloc(n);
     bi->d_counts.alloc(N_LISTS);
     bi->n_done.alloc(1);
     bi->n_done.zero(bi->stream);
@@ -953,7 +905,7 @@ static void drive(BatchImpl& bi, long long max_replays) {
             HIP_OK(hipMemcpyAsync(bi.host_done.data(), bi.done_flag.p, n, hipMemcpyDeviceToHost, bi.stream));
             HIP_OK(hipStreamSynchronize(bi.stream));
             rebuild_lists(bi);
-            if (2 * bi.cnt[L_INST] <= bi.cap[L_INST]) capture(bi);
+            if (bi.need_capture || 2 * bi.cnt[L_INST] <= bi.cap[L_INST]) capture(bi);
         }
     }
 }
@@ -976,10 +928,7 @@ void batch_run(BatchImpl& bi, crush_batch_stats* st) {
     HIP_OK(hipEventElapsedTime(&ms, e0, e1));
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    hipLaunchKernelGGL(k_collect, dim3((n + 255) / 256), dim3(256), 0, bi.stream, bi.table.p, n, bi.summary.p);
-    bi.host_summary.resize(n);
-    HIP_OK(hipMemcpyAsync(bi.host_summary.data(), bi.summary.p, n * sizeof(BatchSummary), hipMemcpyDeviceToHost,
-                          bi.stream));
+    collect(bi, n);
     HIP_OK(hipMemcpyAsync(&bi.host_t_start, bi.t_start.p, sizeof(unsigned long long), hipMemcpyDeviceToHost,
                           bi.stream));
     HIP_OK(hipStreamSynchronize(bi.stream));

@@ -267,6 +267,59 @@ int32_t crush_batch_result(crush_batch* b, int32_t i, crush_results* out);
 
 void crush_batch_free(crush_batch* b);
 
+/* ---- serving ------------------------------------------------------------
+ * A request loop for delta-v queries: models are submitted one at a time
+ * from any thread, prepared on the submitting thread, admitted to the GPU
+ * as slots and device memory free up (admission by an HBM budget), stepped
+ * together with every other resident request (continuous batching: new
+ * requests join between graph replays of a few steps) and retired at their
+ * end time or at separation (crush_settings.separation_steps).  Each
+ * result is the one crush_run gives for that model alone.  There is no
+ * counterpart in the reference, which solves one model per call. */
+
+typedef struct crush_server crush_server;
+
+typedef struct crush_server_options {
+    double hbm_budget_bytes;      /* device memory for resident requests (0: 16 GiB)   */
+    int32_t max_resident;         /* requests on the device at once (0: 4096)          */
+    int32_t steps_per_graph;      /* steps between admissions (0: 16)                  */
+    int32_t threads_per_instance; /* as crush_batch_options (0: 256)                   */
+    int32_t force_tile;           /* as crush_batch_options (0: 256)                   */
+    double contact_cell_scale;    /* as crush_batch_options (0: 1.0)                   */
+    int32_t max_queued;           /* submissions waiting for admission (0: 1 << 20)    */
+    int32_t reserved0;
+} crush_server_options;
+
+#define CRUSH_SUBMIT_FIELDS 1     /* keep the final u and v for crush_server_poll      */
+
+typedef struct crush_server_stats {
+    int64_t submitted, admitted, completed, failed;
+    int32_t resident, queued;
+    double device_bytes_in_use;
+    double latency_p50_ms;        /* submit to result, over completed requests          */
+    double latency_p99_ms;
+    double queue_p50_ms;          /* submit to admission                                */
+    double busy_ms;               /* driver time with requests resident                 */
+    int64_t element_steps;        /* sum over completed requests                         */
+    int64_t replays;
+} crush_server_stats;
+
+void crush_server_defaults(crush_server_options* opt);
+int32_t crush_server_create(const crush_server_options* opt, crush_server** out);
+/* Queue a model; the model may be freed when this returns.  flags:
+ * CRUSH_SUBMIT_FIELDS.  CRUSH_E_BUSY when the queue is full or the model
+ * can never fit the HBM budget. */
+int32_t crush_server_submit(crush_server* s, const crush_model* model, int32_t flags, int64_t* ticket);
+/* Non-blocking: *done = 1 and out filled (then the ticket is released) when
+ * the request has finished, else *done = 0.  The displacement / velocity
+ * arrays are filled only for CRUSH_SUBMIT_FIELDS requests; out->wall_ms is
+ * the submit-to-result latency and out->device_ms the admit-to-result time. */
+int32_t crush_server_poll(crush_server* s, int64_t ticket, crush_results* out, int32_t* done);
+/* Blocking form of crush_server_poll (timeout_ms <= 0: no limit). */
+int32_t crush_server_wait(crush_server* s, int64_t ticket, crush_results* out, double timeout_ms);
+int32_t crush_server_stats_get(crush_server* s, crush_server_stats* st);
+void crush_server_destroy(crush_server* s);
+
 #ifdef __cplusplus
 }
 #endif
