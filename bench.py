@@ -258,6 +258,97 @@ def bench_sweep(args):
     return out
 
 
+def bench_serve(args):
+    """Poisson arrivals of delta-v requests at each offered rate into one
+    server; latency is measured from the scheduled arrival to the result
+    (host preparation at submit included)."""
+    import threading
+    cases = workloads.serve_mix(args.pool, seed=args.seed)
+    t0 = time.perf_counter()
+    pool = workloads.build_models(cases)
+    t_build = time.perf_counter() - t0
+    srv = ca.Server(hbm_budget_bytes=args.hbm_budget_gib * 2**30, max_resident=args.max_resident,
+                    steps_per_graph=args.steps_per_graph)
+    # warm-up: code objects and first captures
+    for t in [srv.submit(pool[i]) for i in range(min(8, len(pool)))]:
+        srv.wait(t)
+    rng = np.random.default_rng(args.seed + 1)
+    levels = []
+    k_next = 0
+    for rate in [float(x) for x in args.rates.split(",")]:
+        n = args.requests
+        arrive = np.cumsum(rng.exponential(1.0 / rate, size=n))
+        idx = [(k_next + i) % len(pool) for i in range(n)]
+        k_next += n
+        st0 = srv.stats()
+        tickets = [None] * n
+        lock = threading.Lock()
+        nxt = [0]
+        base = time.perf_counter() + 0.05
+
+        def submitter():
+            while True:
+                with lock:
+                    i = nxt[0]
+                    nxt[0] += 1
+                if i >= n:
+                    return
+                delay = base + arrive[i] - time.perf_counter()
+                if delay > 0:
+                    time.sleep(delay)
+                tickets[i] = srv.submit(pool[idx[i]])
+
+        threads = [threading.Thread(target=submitter) for _ in range(min(8, n))]
+        for th in threads:
+            th.start()
+        done_at = [None] * n
+        remaining = set(range(n))
+        while remaining:
+            for i in list(remaining):
+                if tickets[i] is not None and srv.poll(tickets[i]) is not None:
+                    done_at[i] = time.perf_counter()
+                    remaining.discard(i)
+            time.sleep(0.002)
+        for th in threads:
+            th.join()
+        st1 = srv.stats()
+        lat = np.array([done_at[i] - (base + arrive[i]) for i in range(n)]) * 1e3
+        span = max(done_at) - base
+        es = st1.element_steps - st0.element_steps
+        levels.append({"offered_per_s": rate, "requests": n, "solves_per_s": n / span,
+                       "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
+                       "queue_p50_ms": st1.queue_p50_ms, "gelem_steps_per_s": es / span / 1e9})
+        print(json.dumps({"level": levels[-1]}), file=sys.stderr, flush=True)
+    srv.close()
+    best = max(levels, key=lambda x: x["solves_per_s"])
+    pg = None if args.no_plain else plain_gpu(pool)
+    return {
+        "metric": METRIC,
+        "value": best["solves_per_s"],
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": len(levels),
+        "warmup": 8,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded request mix (60% head-on/offset, 25% T-bone, 15% rear-end) of the four tuned TOMLs",
+        "config": {"workload": (f"serving: Poisson arrivals at {args.rates} req/s, {args.requests} requests per rate, "
+                                "speeds U(5,45) mph, h in {0.3,0.15} m, retire at separation"),
+                   "pool": len(pool), "elements": {"min": min(m.n_hexes for m in pool),
+                                                   "max": max(m.n_hexes for m in pool)},
+                   "steps_per_graph": args.steps_per_graph, "hbm_budget_gib": args.hbm_budget_gib},
+        "levels": levels,
+        "plain_gpu": pg,
+        "speedup_vs_plain": None if pg is None else best["solves_per_s"] / pg["value"],
+        "host": {"build_models_s": t_build},
+        "roofline": None,
+        "cpu_baseline": None,
+    }
+
+
 def bench_crash(args):
     m = ca.Model.crash(30.0, 30.0)
     for _ in range(args.warmup):
@@ -299,12 +390,17 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="sweep", choices=["sweep", "crash"])
+    ap.add_argument("--workload", default="sweep", choices=["sweep", "crash", "serve"])
+    ap.add_argument("--rates", default="10,50,200,800", help="serve: offered loads, requests/s")
+    ap.add_argument("--requests", type=int, default=400, help="serve: requests per offered load")
+    ap.add_argument("--pool", type=int, default=256, help="serve: distinct request models")
+    ap.add_argument("--hbm-budget-gib", type=float, default=32.0, help="serve: device memory for resident requests")
+    ap.add_argument("--max-resident", type=int, default=4096, help="serve: requests on the device at once")
     ap.add_argument("--instances", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--threads-per-instance", type=int, default=256)
     ap.add_argument("--force-tile", type=int, default=256)
-    ap.add_argument("--steps-per-graph", type=int, default=32)
+    ap.add_argument("--steps-per-graph", type=int, default=None)
     ap.add_argument("--autotune", action="store_true")
     ap.add_argument("--end-time", type=float, default=0.15,
                     help="simulated time per instance (0.15 s; shorter only for profiling runs)")
@@ -326,6 +422,12 @@ def main():
     if args.copy_probe:
         copy_probe(args.copy_probe)
         return
+    if args.workload == "serve":
+        args.steps_per_graph = 16 if args.steps_per_graph is None else args.steps_per_graph
+        out = bench_serve(args)
+        print(json.dumps(out), flush=True)
+        return
+    args.steps_per_graph = 32 if args.steps_per_graph is None else args.steps_per_graph
     if args.workload == "sweep":
         args.steps = 1 if args.steps is None else args.steps
         args.warmup = 1 if args.warmup is None else args.warmup

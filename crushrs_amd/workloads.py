@@ -13,7 +13,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import MPH, Model, renamed, vehicle_file
+from . import MPH, Model, rear_profile, renamed, side_profile, vehicle_file
 
 SWEEP_VEHICLES = ("neon_1996", "silverado_2007", "expedition_1999", "navigator_1999")
 
@@ -62,3 +62,67 @@ def honeycomb_count(model: Model) -> int:
     parts = np.ctypeslib.as_array(v.hex_part, shape=(v.n_hexes,))
     kinds = np.array([v.materials[p].model for p in range(v.n_parts)], dtype=np.int32)
     return int(np.count_nonzero(kinds[parts] == 1))
+
+
+# Rear crush coefficients for vehicles whose TOML has no [rear] section:
+# the front curve scaled by this factor (the tuned TOMLs carry none, and
+# rear_profile needs one, src/vehicle.rs:271-316).
+SYNTHETIC_REAR_FACTOR = 0.7
+
+
+def with_rear(v):
+    """A copy of vehicle v with synthetic rear coefficients if it has none."""
+    out = renamed(v, v.name.decode())
+    if not out.has_rear_curve:
+        out.has_rear_curve = 1
+        out.rear_curve[0] = SYNTHETIC_REAR_FACTOR * v.curve[0]
+        out.rear_curve[1] = SYNTHETIC_REAR_FACTOR * v.curve[1]
+    return out
+
+
+@dataclass(frozen=True)
+class ServeCase:
+    """One delta-v request: kind 'headon' (with lateral offset), 'tbone'
+    (front of a into the side profile of b) or 'rear' (front of a into the
+    rear profile of b, b moving the same way more slowly)."""
+    kind: str
+    a: str
+    b: str
+    speed_a_mph: float
+    speed_b_mph: float
+    offset: float
+    element_size: float
+    separation_steps: int = 20
+
+    def build(self, cache=None) -> Model:
+        cache = {} if cache is None else cache
+        va = renamed(cache.get(self.a) or vehicle_file(self.a), f"{self.a}_a")
+        vb = renamed(cache.get(self.b) or vehicle_file(self.b), f"{self.b}_b")
+        h = self.element_size
+        if self.kind == "headon":
+            m = Model.headon(va, vb, self.speed_a_mph * MPH, self.speed_b_mph * MPH, offset=self.offset, element_size=h)
+        elif self.kind == "tbone":
+            m = Model.tbone(va, side_profile(vb), self.speed_a_mph * MPH, 0.0, offset=self.offset, element_size=h)
+        else:
+            m = Model.headon(va, rear_profile(with_rear(vb)), self.speed_a_mph * MPH, -self.speed_b_mph * MPH,
+                             element_size=h)
+        return m.set(separation_steps=self.separation_steps)
+
+
+def serve_mix(n: int, seed: int = 0, sizes=(0.3, 0.15)) -> list[ServeCase]:
+    """n seeded requests: 60 % head-on/offset, 25 % T-bone, 15 % rear-end;
+    speeds uniform in 5-45 mph (rear-end: the struck vehicle at a random
+    fraction of the striking speed), element size from `sizes`."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        u = rng.random()
+        kind = "headon" if u < 0.60 else ("tbone" if u < 0.85 else "rear")
+        a, b = (SWEEP_VEHICLES[i] for i in rng.integers(0, len(SWEEP_VEHICLES), size=2))
+        sa = float(rng.uniform(5.0, 45.0))
+        sb = float(rng.uniform(5.0, 45.0)) if kind == "headon" else (float(rng.uniform(0.0, 0.8)) * sa
+                                                                     if kind == "rear" else 0.0)
+        off = float(rng.uniform(0.0, 0.6)) if kind == "headon" else (float(rng.uniform(-0.5, 0.5))
+                                                                      if kind == "tbone" else 0.0)
+        out.append(ServeCase(kind, a, b, sa, sb, off, float(rng.choice(np.asarray(sizes, float)))))
+    return out
