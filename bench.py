@@ -1,6 +1,7 @@
 """Benchmark of the crash-solver core on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload sweep|crash]
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+                    [--workload sweep|calib|large|crash|serve]
                     [--instances N] [--no-cpu] [--no-plain]
 
 Workloads
@@ -9,9 +10,20 @@ Workloads
          lateral offset U(0, 0.6) m, element size 0.3 or 0.2 m, rigid road,
          0.15 s each, solved together by the batched persistent kernel.
          One bench step is one batched solve of every instance to its end.
+  calib  the batched calibration set: N (1024) rigid-barrier tests at
+         35 mph of the four TOMLs at 0.3 m, crush curves perturbed as a
+         calibration driver probes them (yield x LogU(0.8, 1.25), knots x
+         U(0.9, 1.1), rail box force_fraction U(0.5, 0.8), width
+         U(0.4, 0.8) m).  One bench step solves every instance.
+  large  the large single instance: the crash presets remeshed at 0.075 m
+         (76,662 hexes, 84,416 nodes), 8-knot tabulated curves with lock-up,
+         mass_scaling 0.25.  One bench step is one solve.
   crash  plain single solve: `crush crash 30 30` (Silverado vs Neon presets,
          1290 hexes, no road) with the one-model path.  One bench step is
          one solve.
+  serve  Poisson arrivals of delta-v requests (head-on/offset, T-bone,
+         rear-end) into the continuous-batching server, at several offered
+         loads; value is the highest achieved solves/s.
 
 Prints one JSON line.  `value` is solves/s of the workload with every model
 already resident in device memory; preparation and upload are reported
@@ -68,11 +80,19 @@ def cpu_info() -> str:
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(models, elem_steps_per_solve: float, budget_s: float = 20.0, sample_steps: int = 100):
-    """The f64 scalar restatement (oracle) on the host cores: each thread
-    runs `sample_steps` explicit steps of one sampled model at a time; the
-    step rate, less the set-up time of a zero-step call, is converted to
-    solves/s of this workload through its mean element-steps per solve."""
+def cpu_baseline(models, elem_steps_per_solve: float, budget_s: float = 20.0, sample_steps: int = 100,
+                 path: str = "lanes"):
+    """The reference's CPU path restated in C, timed on the host cores.
+
+    path "lanes": the default element path of the reference (Settings.simd =
+    true, src/kernel_simd.rs): honeycomb elements as 8-lane f32 AVX2 blocks,
+    other elements scalar f64.  path "f64": the all-scalar f64 oracle.
+
+    Throughput form of a batch on the CPU: one solve per core, in parallel
+    (as the reference's calibration drivers do).  Each thread runs
+    `sample_steps` explicit steps of one sampled model at a time; the step
+    rate, less the set-up time of a zero-step call, is converted to solves/s
+    of this workload through its mean element-steps per solve."""
     sys.path.insert(0, str(ROOT / "tests"))
     import ctypes as C
     import oracle_ffi as of
@@ -82,6 +102,11 @@ def cpu_baseline(models, elem_steps_per_solve: float, budget_s: float = 20.0, sa
     order = rng.permutation(len(models))
     t_end = time.perf_counter() + budget_s
 
+    def steps(m, n, done):
+        if path == "lanes":
+            return lib.oracle_run_steps_lanes(m.ptr, n, 1, None, None, None, None, None, C.byref(done))
+        return lib.oracle_run_steps(m.ptr, n, None, None, None, None, None, C.byref(done))
+
     def worker(w):
         busy, es, k = 0.0, 0.0, 0
         done = C.c_int32()
@@ -90,9 +115,9 @@ def cpu_baseline(models, elem_steps_per_solve: float, budget_s: float = 20.0, sa
                 break
             m = models[j]
             t0 = time.perf_counter()
-            lib.oracle_run_steps(m.ptr, 0, None, None, None, None, None, C.byref(done))
+            steps(m, 0, done)
             t1 = time.perf_counter()
-            lib.oracle_run_steps(m.ptr, sample_steps, None, None, None, None, None, C.byref(done))
+            steps(m, sample_steps, done)
             t2 = time.perf_counter()
             busy += max((t2 - t1) - (t1 - t0), 1e-9)
             es += m.n_hexes * done.value
@@ -103,14 +128,17 @@ def cpu_baseline(models, elem_steps_per_solve: float, budget_s: float = 20.0, sa
         res = list(ex.map(worker, range(cores)))
     rate = sum(r for r, _ in res)
     n = sum(k for _, k in res)
+    what = ("AVX2 8-lane f32 restatement of the reference's default element path (kernel_simd.rs; "
+            "elastic parts scalar f64), -O3 -mavx2 -mfma" if path == "lanes" else
+            "f64 scalar restatement of the reference's solver (kernel.rs path)")
     return {
         "value": rate / elem_steps_per_solve,
         "unit": "solves/s",
         "cores": cores,
         "kind": "port",
-        "sample": (f"f64 scalar restatement of the reference's solver (kernel.rs path), {n} sampled models x "
-                   f"{sample_steps} steps on {cores} threads of {cpu_info()}; {rate / 1e6:.2f} M element-steps/s "
-                   f"converted with this workload's mean {elem_steps_per_solve:.3g} element-steps per solve"),
+        "sample": (f"{what}; one solve per thread, {n} sampled models x {sample_steps} steps on {cores} threads of "
+                   f"{cpu_info()}; {rate / 1e6:.2f} M element-steps/s converted with this workload's mean "
+                   f"{elem_steps_per_solve:.3g} element-steps per solve"),
         "gelem_steps_per_s": rate / 1e9,
     }
 
@@ -159,9 +187,21 @@ def copy_probe(nbytes: int) -> None:
 
 
 def bench_sweep(args):
+    """The batched workloads: the head-on/offset sweep and the barrier
+    calibration set, solved together by the batched path."""
     t0 = time.perf_counter()
-    cases = workloads.headon_sweep(args.instances, seed=args.seed, end_time=args.end_time)
-    models = workloads.build_models(cases)
+    if args.workload == "calib":
+        cases = workloads.calibration_set(args.instances, seed=args.seed)
+        desc = (f"batched calibration set: {args.instances} rigid-barrier tests at 35 mph of the four tuned TOMLs, "
+                "0.3 m hexes, yield x LogU(0.8,1.25), force-table knots x U(0.9,1.1), rail-box force_fraction "
+                "U(0.5,0.8) and width U(0.4,0.8) m, 0.15 s")
+        data = "synthetic: seeded rigid-barrier tests of the reference's four vehicle TOMLs with perturbed crush curves"
+    else:
+        cases = workloads.headon_sweep(args.instances, seed=args.seed, end_time=args.end_time)
+        desc = (f"batched sweep: {args.instances} head-on/offset pairs (neon/silverado/expedition/navigator TOMLs), "
+                f"speeds U(10,40) mph, offset U(0,0.6) m, h in {{0.3,0.2}} m, rigid road, {args.end_time} s")
+        data = "synthetic: seeded head-on/offset pairs of the reference's four vehicle TOMLs, generated meshes"
+    models = workloads.build_cases(cases)
     t_build = time.perf_counter() - t0
     n_hc = np.array([workloads.honeycomb_count(m) for m in models])
     E = np.array([m.n_hexes for m in models])
@@ -223,10 +263,9 @@ def bench_sweep(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: seeded head-on/offset pairs of the reference's four vehicle TOMLs, generated meshes",
+        "data": data,
         "config": {
-            "workload": (f"batched sweep: {n} head-on/offset pairs (neon/silverado/expedition/navigator TOMLs), "
-                         f"speeds U(10,40) mph, offset U(0,0.6) m, h in {{0.3,0.2}} m, rigid road, {args.end_time} s"),
+            "workload": desc,
             "instances": n, "seed": args.seed,
             "elements": {"min": int(E.min()), "max": int(E.max()), "mean": float(E.mean())},
             "solver_steps": {"min": int(steps.min()), "max": int(steps.max()), "mean": float(steps.mean())},
@@ -250,9 +289,12 @@ def bench_sweep(args):
         out["plain_gpu"] = pg
         out["speedup_vs_plain"] = value / pg["value"]
     if not args.no_cpu:
-        cb = cpu_baseline(models, es / n)
+        cb = cpu_baseline(models, es / n, budget_s=15.0)
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = value / cb["value"]
+        f64 = cpu_baseline(models, es / n, budget_s=8.0, path="f64")
+        out["cpu_f64_oracle"] = f64
+        out["speedup_vs_cpu_f64"] = value / f64["value"]
     else:
         out["cpu_baseline"] = None
     return out
@@ -349,6 +391,101 @@ def bench_serve(args):
     }
 
 
+def bench_large(args):
+    """The large single instance (76,662 hexes at 0.075 m): the batched
+    path with one instance (graph-replayed steps), the one-model path, and
+    the lane CPU path with OpenMP threads inside the solve."""
+    import ctypes as C
+    t0 = time.perf_counter()
+    m = workloads.large_pair(end_time=args.end_time)
+    t_build = time.perf_counter() - t0
+    n_hc = workloads.honeycomb_count(m)
+    opt = ca.batch_options(threads_per_instance=args.threads_per_instance, force_tile=args.force_tile,
+                           steps_per_graph=args.steps_per_graph, autotune=int(args.autotune))
+    batch = ca.Batch([m], opt)
+    for _ in range(args.warmup):
+        batch.run()
+        batch.reset()
+    wall = []
+    for k in range(args.steps):
+        if k:
+            batch.reset()
+        wall.append(batch.run().wall_ms)
+    r = batch.result(0, fields=False)
+    steps = r.steps
+    t_run = sum(wall) / 1e3
+    value = args.steps / t_run
+    batch.time_kernels(True)
+    batch.reset()
+    tk = batch.run()
+    batch.time_kernels(False)
+    k_launches = max(1, tk.launches)
+    avg_ms = tk.k_force_hc_ms / k_launches
+    bytes_per_launch = B_HC * tk.hc_element_steps / k_launches
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    es = float(m.n_hexes * steps)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * t_run / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: the reference's crash presets remeshed, generated meshes",
+        "config": {"workload": ("large single instance: Silverado vs Neon presets remeshed at 0.075 m, +-30 mph, "
+                                "no road, 8-knot tabulated crush curves with lock-up, mass_scaling 0.25, "
+                                f"{args.end_time} s"),
+                   "elements": m.n_hexes, "nodes": m.n_nodes, "honeycomb": n_hc, "solver_steps": int(steps),
+                   "force_tile": batch.options.force_tile, "steps_per_graph": batch.options.steps_per_graph},
+        "gelem_steps_per_s": es * args.steps / t_run / 1e9,
+        "us_per_solver_step": 1e6 * t_run / args.steps / max(steps, 1),
+        "delta_v": [g.delta_v[0] for g in r.groups],
+        "host": {"build_model_s": t_build},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": "k_force_hc",
+                     "launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "bytes_per_element_step": B_HC},
+        "kernels": {"ms": {name: tk.k_ms[j] for j, name in enumerate(KERNELS)}, "launches_each": tk.launches},
+    }
+    if not args.no_plain:
+        s0 = time.perf_counter()
+        pr = ca.run(m)
+        tp = time.perf_counter() - s0
+        out["plain_gpu"] = {"value": 1.0 / tp, "unit": "solves/s", "steps": int(pr.steps),
+                            "gelem_steps_per_s": m.n_hexes * pr.steps / tp / 1e9}
+        out["speedup_vs_plain"] = value * tp
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle_ffi as of
+        lib = of.load()
+        cores = max(1, min(os.cpu_count() or 1, 16))
+        done = C.c_int32()
+        n_s = 40
+        s0 = time.perf_counter()
+        lib.oracle_run_steps_lanes(m.ptr, 0, cores, None, None, None, None, None, C.byref(done))
+        s1 = time.perf_counter()
+        lib.oracle_run_steps_lanes(m.ptr, n_s, cores, None, None, None, None, None, C.byref(done))
+        s2 = time.perf_counter()
+        per_step = max((s2 - s1) - (s1 - s0), 1e-9) / max(done.value, 1)
+        out["cpu_baseline"] = {
+            "value": 1.0 / (per_step * steps), "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": (f"AVX2 8-lane f32 restatement of the reference's default element path, {done.value} steps of "
+                       f"this model with {cores} OpenMP threads in the element pass (the rest of the step "
+                       f"sequential, as in the reference) on {cpu_info()}; {per_step * 1e3:.1f} ms per step, set-up "
+                       f"({s1 - s0:.1f} s) excluded, scaled to the GPU's {steps} steps"),
+            "gelem_steps_per_s": m.n_hexes / per_step / 1e9,
+        }
+        out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    else:
+        out["cpu_baseline"] = None
+    return out
+
+
 def bench_crash(args):
     m = ca.Model.crash(30.0, 30.0)
     for _ in range(args.warmup):
@@ -390,13 +527,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="sweep", choices=["sweep", "crash", "serve"])
+    ap.add_argument("--workload", default="sweep", choices=["sweep", "calib", "large", "crash", "serve"])
     ap.add_argument("--rates", default="10,50,200,800", help="serve: offered loads, requests/s")
     ap.add_argument("--requests", type=int, default=400, help="serve: requests per offered load")
     ap.add_argument("--pool", type=int, default=256, help="serve: distinct request models")
     ap.add_argument("--hbm-budget-gib", type=float, default=32.0, help="serve: device memory for resident requests")
     ap.add_argument("--max-resident", type=int, default=4096, help="serve: requests on the device at once")
-    ap.add_argument("--instances", type=int, default=4096)
+    ap.add_argument("--instances", type=int, default=None, help="sweep: 4096, calib: 1024")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--threads-per-instance", type=int, default=256)
     ap.add_argument("--force-tile", type=int, default=256)
@@ -428,10 +565,16 @@ def main():
         print(json.dumps(out), flush=True)
         return
     args.steps_per_graph = 32 if args.steps_per_graph is None else args.steps_per_graph
-    if args.workload == "sweep":
+    if args.workload in ("sweep", "calib"):
+        if args.instances is None:
+            args.instances = 4096 if args.workload == "sweep" else 1024
         args.steps = 1 if args.steps is None else args.steps
         args.warmup = 1 if args.warmup is None else args.warmup
         out = bench_sweep(args)
+    elif args.workload == "large":
+        args.steps = 3 if args.steps is None else args.steps
+        args.warmup = 1 if args.warmup is None else args.warmup
+        out = bench_large(args)
     else:
         args.steps = 20 if args.steps is None else args.steps
         args.warmup = 2 if args.warmup is None else args.warmup

@@ -13,7 +13,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from . import MPH, Model, rear_profile, renamed, side_profile, vehicle_file
+from . import MPH, Model, rear_profile, renamed, side_profile, vehicle, vehicle_file
 
 SWEEP_VEHICLES = ("neon_1996", "silverado_2007", "expedition_1999", "navigator_1999")
 
@@ -126,3 +126,102 @@ def serve_mix(n: int, seed: int = 0, sizes=(0.3, 0.15)) -> list[ServeCase]:
                                                                       if kind == "tbone" else 0.0)
         out.append(ServeCase(kind, a, b, sa, sb, off, float(rng.choice(np.asarray(sizes, float)))))
     return out
+
+
+# ---------------------------------------------------------------- large pair
+
+# Crush distances (m) of the 7-knot force tables given to the two presets of
+# the large single instance.  The forces are the presets' own bilinear
+# curves, F = yield_force + stiffness * x (src/vehicle.rs:621-633), sampled
+# at these distances; Vehicle::crush_material_axial then turns each table
+# into a yield-vs-compaction curve and adds the lock-up knot
+# [1.4, sigma(1.4)] with slope + E (src/vehicle.rs:478-489, 502-511), for 8
+# knots in all.
+LARGE_KNOTS_M = (0.0, 0.1, 0.2, 0.35, 0.5, 0.7, 0.9)
+
+
+def tabulated(v, knots_m=LARGE_KNOTS_M):
+    """A copy of vehicle v whose bilinear crush curve is given as a force
+    table sampled at knots_m."""
+    out = renamed(v, v.name.decode())
+    out.n_force_table = len(knots_m)
+    out.n_compaction_map = 0
+    for i, x in enumerate(knots_m):
+        out.force_table[i][0] = x
+        out.force_table[i][1] = v.curve[0] + v.curve[1] * x
+    return out
+
+
+def large_pair(element_size: float = 0.075, speed_mph: float = 30.0, end_time: float = 0.15) -> Model:
+    """The large single instance: the `crash` presets (Silverado 2622 kg vs
+    Neon 1354 kg) remeshed uniformly at element_size, head-on at +-speed,
+    no road, mass_scaling 0.25, each crush curve an 8-knot table with
+    lock-up.  At 0.075 m: 76,662 hexes (78x27x25 + 58x23x18) and 84,416
+    nodes."""
+    a = tabulated(vehicle("silverado"))
+    b = renamed(tabulated(vehicle("neon")), "neon")
+    m = Model.headon(a, b, speed_mph * MPH, speed_mph * MPH, clearance=None, element_size=element_size,
+                     end_time=end_time)
+    return m.set(mass_scaling=0.25)
+
+
+# ---------------------------------------------------------- calibration set
+
+CALIB_VEHICLES = SWEEP_VEHICLES
+
+
+@dataclass(frozen=True)
+class BarrierCase:
+    """One rigid-barrier test (barrier_model, src/vehicle.rs:951-973) of a
+    vehicle whose crush curve is perturbed the way a calibration driver
+    probes it: the whole curve scaled by yield_scale, each force-table knot
+    by its own factor, and the rail box (vehicles with a crush zone and a
+    rail box) given force_fraction and width."""
+    vehicle: str
+    yield_scale: float
+    knot_scales: tuple
+    rail_fraction: float
+    rail_width: float
+    element_size: float = 0.3
+    speed_mph: float = 35.0
+    end_time: float = 0.15
+
+    def build(self, cache=None) -> Model:
+        cache = {} if cache is None else cache
+        v = renamed(cache.get(self.vehicle) or vehicle_file(self.vehicle), self.vehicle)
+        if v.n_force_table > 0:
+            for i in range(v.n_force_table):
+                v.force_table[i][1] *= self.yield_scale * self.knot_scales[i]
+        else:
+            v.curve[0] *= self.yield_scale
+        if v.has_rail_box:
+            v.rail_box_force_fraction = self.rail_fraction
+            v.rail_box_width = self.rail_width
+        return Model.barrier(v, element_size=self.element_size, speed=self.speed_mph * MPH, end_time=self.end_time)
+
+
+def calibration_set(n: int, seed: int = 0) -> list[BarrierCase]:
+    """n seeded barrier tests at 35 mph, 0.3 m elements: vehicle drawn from
+    the four tuned TOMLs, yield scale log-uniform in [0.8, 1.25], knots
+    uniform in [0.9, 1.1], rail-box force_fraction U(0.5, 0.8) and width
+    U(0.4, 0.8) m."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        name = CALIB_VEHICLES[int(rng.integers(0, len(CALIB_VEHICLES)))]
+        ys = float(np.exp(rng.uniform(np.log(0.8), np.log(1.25))))
+        ks = tuple(float(x) for x in rng.uniform(0.9, 1.1, size=8))
+        out.append(BarrierCase(name, ys, ks, float(rng.uniform(0.5, 0.8)), float(rng.uniform(0.4, 0.8))))
+    return out
+
+
+def build_cases(cases, threads: int = 16) -> list[Model]:
+    """Build any list of cases with a .build(cache) method on host threads."""
+    names = set()
+    for c in cases:
+        for attr in ("a", "b", "vehicle"):
+            if hasattr(c, attr):
+                names.add(getattr(c, attr))
+    cache = {name: vehicle_file(name) for name in names}
+    with ThreadPoolExecutor(max_workers=max(1, threads)) as ex:
+        return list(ex.map(lambda c: c.build(cache), cases))

@@ -7,6 +7,7 @@
  */
 #define _POSIX_C_SOURCE 199309L
 #include "crush_oracle.h"
+#include "crush_avx2.h"
 
 #include <float.h>
 #include <math.h>
@@ -832,6 +833,16 @@ typedef struct {
     int* kind;
     double* knots;  /* [24 * parts] */
     double* cap;    /* [3 * parts] */
+    /* lane path (Settings.simd = true): honeycomb elements in f32 blocks of
+     * 8, the other elements scalar f64 (src/kernel.rs:76-90) */
+    lane_blocks* lanes;
+    int* lane_ids;   /* element of the k-th lane slot */
+    int n_lane;
+    int* scalar_ids; /* elements on the scalar path, in element order */
+    int n_scalar;
+    double* scalar_out; /* [24 * n_scalar] */
+    int* scalar_ok;
+    int threads;
 } elements_t;
 
 static double wave_speed(const crush_material* m) {
@@ -844,6 +855,8 @@ static void elements_free(elements_t* el) {
     free(el->grad); free(el->vol); free(el->gamma); free(el->H); free(el->lumped);
     free(el->dt_factor); free(el->state); free(el->eroded); free(el->kind); free(el->knots);
     free(el->cap);
+    lane_blocks_free(el->lanes);
+    free(el->lane_ids); free(el->scalar_ids); free(el->scalar_out); free(el->scalar_ok);
 }
 
 /* Elements::build: src/kernel.rs:53-91 (scalar path). */
@@ -890,9 +903,41 @@ static int elements_build(const crush_model* m, elements_t* el) {
     return CRUSH_OK;
 }
 
+/* Elements::build with simd = true (src/kernel.rs:76-90) and Blocks::build
+ * (src/kernel_simd.rs:523-546): honeycomb elements, in element order, are
+ * packed into lane blocks; every other element stays on the scalar path. */
+static int elements_build_lanes(const crush_model* m, elements_t* el, int threads) {
+    int ne = el->e;
+    el->threads = threads;
+    el->lane_ids = (int*)malloc(sizeof(int) * (size_t)(ne + 1));
+    el->scalar_ids = (int*)malloc(sizeof(int) * (size_t)(ne + 1));
+    for (int e = 0; e < ne; ++e) {
+        if (el->kind[e] == CRUSH_MAT_HONEYCOMB)
+            el->lane_ids[el->n_lane++] = e;
+        else
+            el->scalar_ids[el->n_scalar++] = e;
+    }
+    el->scalar_out = (double*)malloc(sizeof(double) * 24 * (size_t)(el->n_scalar + 1));
+    el->scalar_ok = (int*)malloc(sizeof(int) * (size_t)(el->n_scalar + 1));
+    el->lanes = lane_blocks_create(el->n_lane);
+    if (!el->lanes) return CRUSH_E_OOM;
+    for (int k = 0; k < el->n_lane; ++k) {
+        int e = el->lane_ids[k];
+        int part = m->hex_part[e];
+        lane_blocks_set(el->lanes, k, m->hexes + 8 * (size_t)e, el->grad + 24 * (size_t)e, el->vol[e],
+                        el->gamma + 32 * (size_t)e, el->H + 144 * (size_t)e,
+                        m->materials[part].youngs_modulus, el->knots + 24 * part, el->cap + 3 * part,
+                        el->state + 25 * (size_t)e, el->eroded[e]);
+    }
+    return CRUSH_OK;
+}
+
 /* Elements::compute_forces (scalar): src/kernel.rs:160-204.  Returns the
  * number of newly eroded elements. */
+static int compute_forces_lanes(const crush_model* m, elements_t* el, const double* u, double* global);
+
 static int compute_forces(const crush_model* m, elements_t* el, const double* u, double* global) {
+    if (el->lanes) return compute_forces_lanes(m, el, u, global);
     int newly = 0;
     for (int e = 0; e < el->e; ++e) {
         if (el->eroded[e]) continue;
@@ -915,6 +960,54 @@ static int compute_forces(const crush_model* m, elements_t* el, const double* u,
             ++newly;
             continue;
         }
+        for (int i = 0; i < 8; ++i)
+            for (int d = 0; d < 3; ++d) global[3 * h[i] + d] += out[3 * i + d];
+    }
+    return newly;
+}
+
+/* Elements::compute_forces with simd = true (src/kernel.rs:160-204): the
+ * lane blocks first (forces, scatter, erosion mirrored back), then the
+ * scalar elements computed independently (in parallel when threads > 1)
+ * and scattered in element order. */
+static int compute_forces_lanes(const crush_model* m, elements_t* el, const double* u, double* global) {
+    int newly = lane_blocks_forces(el->lanes, m->nodes, u, global, el->threads);
+    for (int k = 0; k < el->n_lane; ++k) el->eroded[el->lane_ids[k]] = (uint8_t)lane_blocks_eroded(el->lanes, k);
+    int ns = el->n_scalar;
+#pragma omp parallel for schedule(dynamic, 32) num_threads(el->threads > 1 ? el->threads : 1) if (el->threads > 1)
+    for (int k = 0; k < ns; ++k) {
+        int e = el->scalar_ids[k];
+        double* out = el->scalar_out + 24 * (size_t)k;
+        el->scalar_ok[k] = 1;
+        if (el->eroded[e]) {
+            memset(out, 0, 24 * sizeof(double));
+            continue;
+        }
+        double x[24], ul[24];
+        const int32_t* h = m->hexes + 8 * (size_t)e;
+        for (int i = 0; i < 8; ++i)
+            for (int d = 0; d < 3; ++d) {
+                x[3 * i + d] = m->nodes[3 * h[i] + d] + u[3 * h[i] + d];
+                ul[3 * i + d] = u[3 * h[i] + d];
+            }
+        int part = m->hex_part[e];
+        const crush_material* mat = &m->materials[part];
+        el->scalar_ok[k] = oracle_element_force(x, ul, el->grad + 24 * (size_t)e, el->vol[e],
+                                                el->gamma + 32 * (size_t)e, el->H + 144 * (size_t)e,
+                                                el->kind[e], mat->youngs_modulus, mat->poisson_ratio,
+                                                el->knots + 24 * part, el->cap + 3 * part,
+                                                el->state + 25 * (size_t)e, out);
+    }
+    for (int k = 0; k < ns; ++k) {
+        int e = el->scalar_ids[k];
+        if (!el->scalar_ok[k]) {
+            el->eroded[e] = 1;
+            ++newly;
+            continue;
+        }
+        if (el->eroded[e]) continue;
+        const double* out = el->scalar_out + 24 * (size_t)k;
+        const int32_t* h = m->hexes + 8 * (size_t)e;
         for (int i = 0; i < 8; ++i)
             for (int d = 0; d < 3; ++d) global[3 * h[i] + d] += out[3 * i + d];
     }
@@ -979,14 +1072,16 @@ static double group_crush(const crush_group* g, const double* u) {
     return fabs(u[3 * g->accel_node + g->axis] - s / g->n_front);
 }
 
-/* solver::run: src/solver.rs:190-381 with Settings.simd = false. */
-static int solve(const crush_model* m, crush_results* out, taps_t* taps) {
+/* solver::run: src/solver.rs:190-381; simd selects the element path
+ * (Settings.simd, src/model.rs:100-102). */
+static int solve(const crush_model* m, crush_results* out, taps_t* taps, int simd, int threads) {
     double t_start = now_ms();
     const crush_settings* s = &m->settings;
     int nn = m->n_nodes, n = 3 * nn;
     elements_t el;
     memset(&el, 0, sizeof el);
     int rc = elements_build(m, &el);
+    if (!rc && simd) rc = elements_build_lanes(m, &el, threads);
     if (rc) {
         elements_free(&el);
         return rc;
@@ -1203,12 +1298,25 @@ done:
     return rc;
 }
 
-int oracle_run(const crush_model* model, crush_results* out) { return solve(model, out, NULL); }
+int oracle_run(const crush_model* model, crush_results* out) { return solve(model, out, NULL, 0, 1); }
+
+int oracle_run_lanes(const crush_model* model, int32_t threads, crush_results* out) {
+    return solve(model, out, NULL, 1, threads);
+}
+
+int oracle_run_steps_lanes(const crush_model* model, int32_t n_steps, int32_t threads, double* f_int,
+                           double* f_ext, int32_t* hit, uint8_t* eroded, double* dt,
+                           int32_t* steps_done) {
+    taps_t t = {n_steps, f_int, f_ext, hit, eroded, dt, 0};
+    int rc = solve(model, NULL, &t, 1, threads);
+    if (steps_done) *steps_done = t.steps_done;
+    return rc;
+}
 
 int oracle_run_steps(const crush_model* model, int32_t n_steps, double* f_int, double* f_ext,
                      int32_t* hit, uint8_t* eroded, double* dt, int32_t* steps_done) {
     taps_t t = {n_steps, f_int, f_ext, hit, eroded, dt, 0};
-    int rc = solve(model, NULL, &t);
+    int rc = solve(model, NULL, &t, 0, 1);
     if (steps_done) *steps_done = t.steps_done;
     return rc;
 }

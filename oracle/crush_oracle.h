@@ -90,6 +90,18 @@ int oracle_run(const crush_model* model, crush_results* out);
 int oracle_run_steps(const crush_model* model, int32_t n_steps, double* f_int, double* f_ext,
                      int32_t* hit, uint8_t* eroded, double* dt, int32_t* steps_done);
 
+/* The whole solve with the reference's default element path
+ * (Settings.simd = true): honeycomb elements through the f32 AVX2 lane
+ * blocks of crush_avx2.h, the rest scalar f64.  threads > 1 spreads the
+ * element pass over OpenMP threads (results do not depend on it).  This is
+ * the timed CPU baseline. */
+int oracle_run_lanes(const crush_model* model, int32_t threads, crush_results* out);
+
+/* oracle_run_steps on the lane path. */
+int oracle_run_steps_lanes(const crush_model* model, int32_t n_steps, int32_t threads, double* f_int,
+                           double* f_ext, int32_t* hit, uint8_t* eroded, double* dt,
+                           int32_t* steps_done);
+
 /* One contact evaluation (ContactRuntime::apply, src/contact.rs:92-160)
  * over the contacts of `model` at displacement u, adding into force[3N];
  * hit[] as in oracle_run_steps.  Returns the number of penetrating nodes. */

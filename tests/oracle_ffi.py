@@ -53,6 +53,22 @@ def load() -> C.CDLL:
         lib.oracle_run.argtypes = [C.POINTER(_abi.ModelView), C.POINTER(_abi.Results)]
         lib.oracle_run_steps.argtypes = [C.POINTER(_abi.ModelView), C.c_int32, f64p, f64p, i32p, u8p, f64p, i32p]
         lib.oracle_contact_apply.argtypes = [C.POINTER(_abi.ModelView), f64p, f64p, C.c_double, f64p, i32p]
+        lib.oracle_run_lanes.argtypes = [C.POINTER(_abi.ModelView), C.c_int32, C.POINTER(_abi.Results)]
+        lib.oracle_run_steps_lanes.argtypes = [C.POINTER(_abi.ModelView), C.c_int32, C.c_int32, f64p, f64p, i32p,
+                                               u8p, f64p, i32p]
+        lib.lane_blocks_create.argtypes = [C.c_int]
+        lib.lane_blocks_create.restype = C.c_void_p
+        lib.lane_blocks_free.argtypes = [C.c_void_p]
+        lib.lane_blocks_free.restype = None
+        lib.lane_blocks_set.argtypes = [C.c_void_p, C.c_int, i32p, f64p, C.c_double, f64p, f64p, C.c_double, f64p,
+                                        f64p, f64p, C.c_int]
+        lib.lane_blocks_set.restype = None
+        lib.lane_blocks_forces.argtypes = [C.c_void_p, f64p, f64p, f64p, C.c_int]
+        lib.lane_blocks_eroded.argtypes = [C.c_void_p, C.c_int]
+        lib.lane_blocks_state.argtypes = [C.c_void_p, C.c_int, f64p]
+        lib.lane_blocks_state.restype = None
+        lib.lane_blocks_out.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_float)]
+        lib.lane_blocks_out.restype = None
         _lib = lib
     return _lib
 
@@ -73,6 +89,15 @@ def run(model):
     r, u, v, m, g = ca.alloc_results(model.n_nodes, model.view.n_groups)
     rc = load().oracle_run(model.ptr, C.byref(r))
     assert rc == 0, f"oracle_run failed: {rc}"
+    return ca.to_results(r, u, v, m, g)
+
+
+def run_lanes(model, threads: int = 1):
+    """oracle_run_lanes (the reference's default f32 AVX2 element path)."""
+    import crushrs_amd as ca
+    r, u, v, m, g = ca.alloc_results(model.n_nodes, model.view.n_groups)
+    rc = load().oracle_run_lanes(model.ptr, threads, C.byref(r))
+    assert rc == 0, f"oracle_run_lanes failed: {rc}"
     return ca.to_results(r, u, v, m, g)
 
 
