@@ -359,7 +359,12 @@ def bench_serve(args):
         es = st1.element_steps - st0.element_steps
         levels.append({"offered_per_s": rate, "requests": n, "solves_per_s": n / span,
                        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99))},
-                       "queue_p50_ms": st1.queue_p50_ms, "gelem_steps_per_s": es / span / 1e9})
+                       "queue_p50_ms": st1.queue_p50_ms, "gelem_steps_per_s": es / span / 1e9,
+                       "driver_ms": {"admit": st1.admit_ms - st0.admit_ms, "lists": st1.lists_ms - st0.lists_ms,
+                                     "replay": st1.replay_ms - st0.replay_ms,
+                                     "retire": st1.retire_ms - st0.retire_ms},
+                       "replays": st1.replays - st0.replays, "captures": st1.captures - st0.captures,
+                       "span_ms": span * 1e3})
         print(json.dumps({"level": levels[-1]}), file=sys.stderr, flush=True)
     srv.close()
     best = max(levels, key=lambda x: x["solves_per_s"])

@@ -235,6 +235,11 @@ class ServerStats(C.Structure):
         ("busy_ms", C.c_double),
         ("element_steps", C.c_int64),
         ("replays", C.c_int64),
+        ("captures", C.c_int64),
+        ("admit_ms", C.c_double),
+        ("lists_ms", C.c_double),
+        ("replay_ms", C.c_double),
+        ("retire_ms", C.c_double),
     ]
 
 

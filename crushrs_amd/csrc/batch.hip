@@ -122,7 +122,23 @@ ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     o.fixed = b.put(P.fixed);
     o.n2e_ptr = b.put(P.n2e_ptr);
     o.n2e_idx = b.put(P.n2e_idx);
-    o.cdesc = b.put(P.cdesc);
+    // face-grid region of each contact: [starts: cells + 1][entries], plus a
+    // cursor array of `cells` for grids built in device memory (contacts
+    // with more faces than the LDS grid is sized for)
+    std::vector<ContactDesc> cdesc = P.cdesc;
+    size_t grid_ints = 0;
+    for (ContactDesc& cd : cdesc) {
+        cd.grid_cells = BATCH_GRID_CELLS;
+        cd.grid_entries = BATCH_GRID_ENTRIES;
+        const bool big = cd.n_faces > BATCH_GRID_LARGE;
+        if (big) {
+            cd.grid_cells = std::min(std::max(8 * cd.n_faces, BATCH_GRID_CELLS), 1 << 16);
+            cd.grid_entries = std::min(std::max(64 * cd.n_faces, BATCH_GRID_ENTRIES), 1 << 21);
+        }
+        cd.grid_off = (int32_t)grid_ints;
+        grid_ints += (size_t)cd.grid_cells + 1 + cd.grid_entries + (big ? cd.grid_cells : 0);
+    }
+    o.cdesc = b.put(cdesc);
     o.cn_node = b.put(P.cn_node);
     o.cn_scale = b.put(P.cn_scale);
     o.cf_nodes = b.put(P.cf_nodes);
@@ -159,7 +175,7 @@ ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     o.rec_f = scratch(sizeof(double) * 3 * (size_t)P.n_cnodes);
     o.fh_list = scratch(sizeof(int32_t) * FACE_HITS_CAP * (size_t)P.n_cfaces);
     o.face_force = scratch(sizeof(double) * 3 * (size_t)P.n_cfaces);
-    o.cgrid = scratch(sizeof(int32_t) * GRID_STRIDE * P.cdesc.size());
+    o.cgrid = scratch(sizeof(int32_t) * grid_ints);
     o.cgp = scratch(sizeof(FaceGrid) * P.cdesc.size());
     o.dtp_min = scratch(sizeof(double) * dtp_slots(P));
     o.dtp_sum = scratch(sizeof(double) * dtp_slots(P));
@@ -367,15 +383,21 @@ __global__ __launch_bounds__(B) void k_grid(const InstDev* __restrict__ table, c
     const ContactDesc cd = D.cdesc[ci];
     for (int f = threadIdx.x; f < cd.n_faces; f += B) contact_face(D, cd.face_begin + f);
     __syncthreads();
-    const FaceGrid g = build_face_grid<B, BATCH_GRID_CELLS, BATCH_GRID_ENTRIES>(D, cd, cell_scale, s_start, s_cursor,
-                                                                                 s_entries, s_w, s_tmp);
-    int32_t* gs = D.cgrid + (size_t)ci * GRID_STRIDE;
-    int32_t* ge = gs + BATCH_GRID_CELLS + 1;
+    FaceGrid g = build_face_grid<B>(D, cd, cell_scale, BATCH_GRID_CELLS, BATCH_GRID_ENTRIES, s_start, s_cursor,
+                                    s_entries, s_w, s_tmp);
+    int32_t* gs = D.cgrid + cd.grid_off;
+    int32_t* ge = gs + cd.grid_cells + 1;
     if (!g.overflow) {
         const int ncell = g.dim[0] * g.dim[1] * g.dim[2];
         for (int i = threadIdx.x; i <= ncell; i += B) gs[i] = s_start[i];
         const int total = s_start[ncell];
         for (int i = threadIdx.x; i < total; i += B) ge[i] = s_entries[i];
+    } else if (cd.n_faces > BATCH_GRID_LARGE) {
+        // too many entries for LDS: build the grid again in this contact's
+        // device-memory region, whose capacities grow with its face count
+        __syncthreads();
+        g = build_face_grid<B>(D, cd, cell_scale, cd.grid_cells, cd.grid_entries, gs, ge + cd.grid_entries, ge, s_w,
+                               s_tmp);
     }
     if (threadIdx.x == 0) D.cgp[ci] = g;
 }
@@ -397,8 +419,8 @@ __global__ __launch_bounds__(T) void k_search_tiles(const InstDev* __restrict__ 
     int hit = 0;
     if (q < cd.n_nodes) {
         const FaceGrid g = D.cgp[ci];
-        const int32_t* gs = D.cgrid + (size_t)ci * GRID_STRIDE;
-        hit = contact_node(D, cd, q, g, gs, gs + BATCH_GRID_CELLS + 1, step_dt(c));
+        const int32_t* gs = D.cgrid + cd.grid_off;
+        hit = contact_node(D, cd, q, g, gs, gs + cd.grid_cells + 1, step_dt(c));
     }
     const unsigned long long m = __ballot(hit != 0 && cd.pair);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(&c.pair_hits, (int)__popcll(m));

@@ -315,7 +315,8 @@ CRUSH_HD void sym_eigen3f(const float* a, float* vals, float* vecs) { sym_eigen3
 // f64, where the smallest eigenvalues of b keep their relative accuracy.
 // Kept out of line: rare, and its f64 temporaries would otherwise raise the
 // register budget of every caller.
-__host__ __device__ __attribute__((noinline)) inline void hencky_log_strain_large(const float* hd, float* eps) {
+template <class T>
+__host__ __device__ __attribute__((noinline)) inline void hencky_log_strain_large(const T* hd, T* eps) {
     double bd[9], lam[3], vec[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -333,60 +334,72 @@ __host__ __device__ __attribute__((noinline)) inline void hencky_log_strain_larg
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            eps[3 * i + j] = (float)(e3[0] * vec[i] * vec[j] + e3[1] * vec[3 + i] * vec[3 + j] + e3[2] * vec[6 + i] * vec[6 + j]);
+            eps[3 * i + j] = (T)(e3[0] * vec[i] * vec[j] + e3[1] * vec[3 + i] * vec[3 + j] + e3[2] * vec[6 + i] * vec[6 + j]);
+}
+
+template <class T>
+CRUSH_HD void mul3t(const T* a, const T* b, T* r) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) r[3 * i + j] = a[3 * i] * b[j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
 }
 
 // Hencky elastic Kirchhoff stress (src/material.rs:708-771 with no
 // plasticity): tau = K tr(eps) I + 2G dev(eps) with eps = ln(b)/2, b = F F^T.
 // The small quantity A = b - I = Hd + Hd^T + Hd Hd^T is formed from the
-// displacement gradient Hd = F - I.  For |A| < 0.05 (every elastic element of
-// a vehicle model in practice) eps = ln(I + A)/2 is the matrix series to A^6
-// (truncation below 2e-10 |A|); otherwise principal stretches from an f64
+// displacement gradient Hd = F - I.  For |A| below a threshold (0.05 in f32,
+// 0.01 in f64: every elastic element of a vehicle model in practice)
+// eps = ln(I + A)/2 is the matrix series to A^6 (truncation below 2e-10 |A|
+// in f32, 2e-13 |A| in f64); otherwise principal stretches from an f64
 // eigen-decomposition with eps_a = log1p(mu_a)/2.
-CRUSH_HD void hencky_elastic(const MatParams& mp, const float* hd, float* tau) {
-    float bm[9];
+template <class T>
+CRUSH_HD void hencky_elastic_t(const MatParams& mp, const T* hd, T* tau) {
+    T bm[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
             bm[3 * i + j] = hd[3 * i + j] + hd[3 * j + i] +
                             (hd[3 * i] * hd[3 * j] + hd[3 * i + 1] * hd[3 * j + 1] + hd[3 * i + 2] * hd[3 * j + 2]);
-    float mx = 0.0f;
+    T mx = T(0);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) mx = fmaxf(mx, fabsf(bm[k]));
-    float eps[9];
-    if (mx < 0.05f) {
+    for (int k = 0; k < 9; ++k) mx = fmax(mx, fabs(bm[k]));
+    T eps[9];
+    if (mx < (sizeof(T) == 4 ? T(0.05) : T(0.01))) {
         // ln(I + A) = A (I - A (I/2 - A (I/3 - A (I/4 - A (I/5 - A/6)))))
-        const float c[6] = {1.0f, 0.5f, 1.0f / 3.0f, 0.25f, 0.2f, 1.0f / 6.0f};
-        float acc[9];
+        const T c[6] = {T(1), T(0.5), T(1) / T(3), T(0.25), T(0.2), T(1) / T(6)};
+        T acc[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) acc[k] = ((k == 0 || k == 4 || k == 8) ? c[4] : 0.0f) - c[5] * bm[k];
+        for (int k = 0; k < 9; ++k) acc[k] = ((k == 0 || k == 4 || k == 8) ? c[4] : T(0)) - c[5] * bm[k];
 #pragma unroll
         for (int n = 3; n >= 0; --n) {
-            float t[9];
-            mul3f(bm, acc, t);
+            T t[9];
+            mul3t(bm, acc, t);
 #pragma unroll
-            for (int k = 0; k < 9; ++k) acc[k] = ((k == 0 || k == 4 || k == 8) ? c[n] : 0.0f) - t[k];
+            for (int k = 0; k < 9; ++k) acc[k] = ((k == 0 || k == 4 || k == 8) ? c[n] : T(0)) - t[k];
         }
         // acc = I - A/2 + A^2/3 - ...; ln(I + A) = A acc
-        float l[9];
-        mul3f(bm, acc, l);
+        T l[9];
+        mul3t(bm, acc, l);
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int j = 0; j < 3; ++j) eps[3 * i + j] = 0.25f * (l[3 * i + j] + l[3 * j + i]);
+            for (int j = 0; j < 3; ++j) eps[3 * i + j] = T(0.25) * (l[3 * i + j] + l[3 * j + i]);
     } else {
-        hencky_log_strain_large(hd, eps);
+        hencky_log_strain_large<T>(hd, eps);
     }
-    const float vol = eps[0] + eps[4] + eps[8];
-    const float pressure = mp.bulk * vol;
-    const float g2 = 2.0f * mp.shear;
+    const T vol = eps[0] + eps[4] + eps[8];
+    const T pressure = (T)mp.bulk * vol;
+    const T g2 = T(2) * (T)mp.shear;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const bool dg = (k == 0 || k == 4 || k == 8);
-        tau[k] = g2 * (eps[k] - (dg ? vol * (1.0f / 3.0f) : 0.0f)) + (dg ? pressure : 0.0f);
+        tau[k] = g2 * (eps[k] - (dg ? vol * (T(1) / T(3)) : T(0))) + (dg ? pressure : T(0));
     }
 }
+
+CRUSH_HD void hencky_elastic(const MatParams& mp, const float* hd, float* tau) { hencky_elastic_t<float>(mp, hd, tau); }
 
 // Nodal forces of one element from tau (src/kernel.rs:141-154,
 // src/kernel_simd.rs:445-484): f_i = tau F^-T grad_i V + hourglass.

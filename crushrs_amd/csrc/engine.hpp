@@ -16,7 +16,9 @@ namespace crush {
 
 constexpr int BATCH_GRID_CELLS = 1024;
 constexpr int BATCH_GRID_ENTRIES = 4096;
-constexpr int GRID_STRIDE = BATCH_GRID_CELLS + 1 + BATCH_GRID_ENTRIES;
+// contacts with more faces than this get a device-memory grid region sized
+// to their face count (used when the LDS grid overflows)
+constexpr int BATCH_GRID_LARGE = 128;
 constexpr int NODE_TILE = 256;
 constexpr int CONTACT_TILE = 64;
 

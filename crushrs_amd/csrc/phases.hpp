@@ -97,7 +97,8 @@ struct InstDev {
     double* dtp_min;
     double* dtp_sum;
     // batched path: each contact's face grid in device memory (cell starts
-    // then entries, GRID_STRIDE ints per contact) and its parameters
+    // then entries, in the region ContactDesc::grid_off/grid_cells/
+    // grid_entries describe) and its parameters
     int32_t* cgrid;
     FaceGrid* cgp;
 };
@@ -554,12 +555,15 @@ __device__ void block_scan(int* s, int n, int* s_tmp) {
 }
 
 // Uniform grid over one contact's face candidate boxes, built by the whole
-// block in LDS: cell size = cell_scale x the largest face extent (grown
-// until the grid has at most MAXC cells); face lists per cell in s_entries
-// (order within a cell is irrelevant: the search keeps the lowest index).
-template <int B, int MAXC, int MAXE>
-__device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, double cell_scale, int* s_start,
-                                    int* s_cursor, int* s_entries, double* s_w, int* s_tmp) {
+// block: cell size = cell_scale x the largest face extent (grown until the
+// grid has at most MAXC cells); face lists per cell in s_entries (order
+// within a cell is irrelevant: the search keeps the lowest index).  The
+// arrays are in LDS, or in device memory for grids larger than LDS holds;
+// MAXC and MAXE are their capacities.
+template <int B>
+__device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, double cell_scale, const int MAXC,
+                                    const int MAXE, int* s_start, int* s_cursor, int* s_entries, double* s_w,
+                                    int* s_tmp) {
     const int t = threadIdx.x, F = D.F, fb = cd.face_begin, nf = cd.n_faces;
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY}, sz = 0.0;
     for (int f = t; f < nf; f += B) {

@@ -31,7 +31,11 @@ struct ContactDesc {
     int32_t node_begin, n_nodes;   // into the flattened contact-node arrays
     int32_t face_begin, n_faces;   // into the flattened contact-face arrays
     double stiffness, max_distance, depth_cap;
-    int32_t pair, reserved;
+    int32_t pair;
+    // batched path: this contact's face-grid region in the instance's grid
+    // array (offset in ints) and its capacities in cells and entries
+    int32_t grid_cells;
+    int32_t grid_entries, grid_off;
 };
 
 struct Prepared {

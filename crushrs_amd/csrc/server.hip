@@ -118,6 +118,8 @@ struct ServerImpl {
     int64_t next_ticket = 1, n_submitted = 0, n_admitted = 0, n_completed = 0, n_failed = 0, element_steps = 0;
     std::vector<double> lat, qlat;
     double busy_ms = 0.0;
+    double admit_ms = 0.0, lists_ms = 0.0, replay_ms = 0.0, retire_ms = 0.0;
+    int64_t captures = 0;
     bool stop = false;
     std::string fatal;
     std::thread driver;
@@ -213,16 +215,23 @@ static void driver_loop(ServerImpl* Sp) {
                 S.resident += (int)admitted.size();
                 S.n_admitted += (int64_t)admitted.size();
             }
+            const auto ta = Clock::now();
             for (auto& r : admitted) upload(S, *r);
+            const auto tb = Clock::now();
+            int captured = 0;
             if (!admitted.empty() || lists_dirty) {
                 rebuild_lists(E);
                 // capture again when a list outgrows its grid, or when the
                 // lists have shrunk to a quarter of it
                 bool again = E.need_capture || 4 * E.cnt[L_INST] < E.cap[L_INST];
                 for (int L = 0; L < N_LISTS; ++L) again = again || E.cnt[L] > E.cap[L];
-                if (again && E.cnt[L_INST] > 0) capture(E);
+                if (again && E.cnt[L_INST] > 0) {
+                    capture(E);
+                    captured = 1;
+                }
                 lists_dirty = false;
             }
+            const auto tc = Clock::now();
             if (E.cnt[L_INST] == 0) continue;
             const auto t0 = Clock::now();
             if (!admitted.empty()) dt_init(E);
@@ -230,12 +239,21 @@ static void driver_loop(ServerImpl* Sp) {
             HIP_OK(hipMemcpyAsync(S.flags_host.data(), E.done_flag.p, S.capacity, hipMemcpyDeviceToHost, E.stream));
             HIP_OK(hipStreamSynchronize(E.stream));
             E.replays += 1;
+            const auto td = Clock::now();
             // retire
             std::vector<std::shared_ptr<Request>> finished;
             for (int slot = 0; slot < S.capacity; ++slot)
                 if (S.slot_req[slot] && S.flags_host[slot]) finished.push_back(S.slot_req[slot]);
             for (auto& r : finished) read_result(S, *r);
             const auto t1 = Clock::now();
+            {
+                std::lock_guard<std::mutex> g(S.mu);
+                S.admit_ms += ms_between(ta, tb);
+                S.lists_ms += ms_between(tb, tc);
+                S.replay_ms += ms_between(t0, td);
+                S.retire_ms += ms_between(td, t1);
+                S.captures += captured;
+            }
             if (!finished.empty()) {
                 std::lock_guard<std::mutex> g(S.mu);
                 for (auto& r : finished) {
@@ -413,6 +431,11 @@ void server_stats(ServerImpl& S, crush_server_stats* st) {
     st->busy_ms = S.busy_ms;
     st->element_steps = S.element_steps;
     st->replays = S.eng.replays;
+    st->captures = S.captures;
+    st->admit_ms = S.admit_ms;
+    st->lists_ms = S.lists_ms;
+    st->replay_ms = S.replay_ms;
+    st->retire_ms = S.retire_ms;
 }
 
 }  // namespace crush

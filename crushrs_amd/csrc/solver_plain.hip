@@ -262,7 +262,7 @@ __global__ __launch_bounds__(256) void k_contact_search(InstDev D, double cell_s
     StepCtl* ctl = D.ctl;
     if (ctl->done) return;
     const ContactDesc cd = D.cdesc[D.sblk_contact[blockIdx.x]];
-    const FaceGrid g = build_face_grid<256, PLAIN_GRID_CELLS, PLAIN_GRID_ENTRIES>(D, cd, cell_scale, s_start, s_cursor,
+    const FaceGrid g = build_face_grid<256>(D, cd, cell_scale, PLAIN_GRID_CELLS, PLAIN_GRID_ENTRIES, s_start, s_cursor,
                                                                                    s_entries, s_w, s_tmp);
     const int k = D.sblk_start[blockIdx.x] + threadIdx.x;
     if (k >= cd.n_nodes) return;

@@ -302,6 +302,11 @@ typedef struct crush_server_stats {
     double busy_ms;               /* driver time with requests resident                 */
     int64_t element_steps;        /* sum over completed requests                         */
     int64_t replays;
+    int64_t captures;             /* step graphs captured (lists outgrew the grids)      */
+    double admit_ms;              /* driver time uploading admitted requests             */
+    double lists_ms;              /* driver time rebuilding lists and capturing graphs   */
+    double replay_ms;             /* driver time in replays, through the done-flag read  */
+    double retire_ms;             /* driver time reading the results of retired requests */
 } crush_server_stats;
 
 void crush_server_defaults(crush_server_options* opt);

@@ -936,6 +936,19 @@ static int elements_build_lanes(const crush_model* m, elements_t* el, int thread
  * number of newly eroded elements. */
 static int compute_forces_lanes(const crush_model* m, elements_t* el, const double* u, double* global);
 
+/* Test hook: element forces of the kinds in g_hook_kinds computed by an
+ * outside function (e.g. another implementation's element arithmetic), the
+ * rest of the solve unchanged.  Process-global; not thread-safe. */
+static oracle_element_hook g_hook;
+static void* g_hook_ctx;
+static int g_hook_kinds;
+
+void oracle_set_element_hook(oracle_element_hook fn, void* ctx, int32_t kinds_mask) {
+    g_hook = fn;
+    g_hook_ctx = ctx;
+    g_hook_kinds = kinds_mask;
+}
+
 static int compute_forces(const crush_model* m, elements_t* el, const double* u, double* global) {
     if (el->lanes) return compute_forces_lanes(m, el, u, global);
     int newly = 0;
@@ -950,11 +963,13 @@ static int compute_forces(const crush_model* m, elements_t* el, const double* u,
             }
         int part = m->hex_part[e];
         const crush_material* mat = &m->materials[part];
-        int ok = oracle_element_force(x, ul, el->grad + 24 * (size_t)e, el->vol[e],
-                                      el->gamma + 32 * (size_t)e, el->H + 144 * (size_t)e,
-                                      el->kind[e], mat->youngs_modulus, mat->poisson_ratio,
-                                      el->knots + 24 * part, el->cap + 3 * part,
-                                      el->state + 25 * (size_t)e, out);
+        int ok = (g_hook && (g_hook_kinds & (1 << el->kind[e]))) ? g_hook(g_hook_ctx, e, el->kind[e], ul, out) : -1;
+        if (ok < 0)
+            ok = oracle_element_force(x, ul, el->grad + 24 * (size_t)e, el->vol[e],
+                                            el->gamma + 32 * (size_t)e, el->H + 144 * (size_t)e,
+                                            el->kind[e], mat->youngs_modulus, mat->poisson_ratio,
+                                            el->knots + 24 * part, el->cap + 3 * part,
+                                            el->state + 25 * (size_t)e, out);
         if (!ok) {
             el->eroded[e] = 1;
             ++newly;
@@ -1040,6 +1055,7 @@ typedef struct {
     uint8_t* eroded;
     double* dt;
     int32_t steps_done;
+    double* u;
 } taps_t;
 
 static double now_ms(void) {
@@ -1232,6 +1248,7 @@ static int solve(const crush_model* m, crush_results* out, taps_t* taps, int sim
             if (taps->hit) memcpy(taps->hit + st * total_cn, hits, sizeof(int32_t) * (size_t)total_cn);
             if (taps->eroded) memcpy(taps->eroded + st * m->n_hexes, el.eroded, (size_t)m->n_hexes);
             if (taps->dt) taps->dt[st] = dt;
+            if (taps->u) memcpy(taps->u + st * n, u, sizeof(double) * (size_t)n);
         }
         step += 1;
         t += dt;
@@ -1307,7 +1324,7 @@ int oracle_run_lanes(const crush_model* model, int32_t threads, crush_results* o
 int oracle_run_steps_lanes(const crush_model* model, int32_t n_steps, int32_t threads, double* f_int,
                            double* f_ext, int32_t* hit, uint8_t* eroded, double* dt,
                            int32_t* steps_done) {
-    taps_t t = {n_steps, f_int, f_ext, hit, eroded, dt, 0};
+    taps_t t = {n_steps, f_int, f_ext, hit, eroded, dt, 0, NULL};
     int rc = solve(model, NULL, &t, 1, threads);
     if (steps_done) *steps_done = t.steps_done;
     return rc;
@@ -1315,8 +1332,16 @@ int oracle_run_steps_lanes(const crush_model* model, int32_t n_steps, int32_t th
 
 int oracle_run_steps(const crush_model* model, int32_t n_steps, double* f_int, double* f_ext,
                      int32_t* hit, uint8_t* eroded, double* dt, int32_t* steps_done) {
-    taps_t t = {n_steps, f_int, f_ext, hit, eroded, dt, 0};
+    taps_t t = {n_steps, f_int, f_ext, hit, eroded, dt, 0, NULL};
     int rc = solve(model, NULL, &t, 0, 1);
+    if (steps_done) *steps_done = t.steps_done;
+    return rc;
+}
+
+int oracle_run_displacements(const crush_model* model, int32_t n_steps, int32_t simd, double* u,
+                             int32_t* steps_done) {
+    taps_t t = {n_steps, NULL, NULL, NULL, NULL, NULL, 0, u};
+    int rc = solve(model, NULL, &t, simd ? 1 : 0, 1);
     if (steps_done) *steps_done = t.steps_done;
     return rc;
 }

@@ -102,6 +102,21 @@ int oracle_run_steps_lanes(const crush_model* model, int32_t n_steps, int32_t th
                            double* f_ext, int32_t* hit, uint8_t* eroded, double* dt,
                            int32_t* steps_done);
 
+/* Test hook: when set, oracle_run / oracle_run_steps (the scalar path)
+ * take the force of every element whose material kind k has bit (1 << k)
+ * in kinds_mask from fn(ctx, element, kind, u[24] of its nodes, out[24]),
+ * which returns 1, 0 when the element inverts (out zeroed), or -1 to
+ * leave that element to the oracle's own arithmetic.  Lets an
+ * alternative element arithmetic run inside the oracle's solve loop.
+ * Process-global; NULL clears it. */
+typedef int (*oracle_element_hook)(void* ctx, int32_t element, int32_t kind, const double* u24, double* out24);
+void oracle_set_element_hook(oracle_element_hook fn, void* ctx, int32_t kinds_mask);
+
+/* Displacement after each of the first n_steps steps, u[s*3N..]; simd
+ * selects the lane path. */
+int oracle_run_displacements(const crush_model* model, int32_t n_steps, int32_t simd, double* u,
+                             int32_t* steps_done);
+
 /* One contact evaluation (ContactRuntime::apply, src/contact.rs:92-160)
  * over the contacts of `model` at displacement u, adding into force[3N];
  * hit[] as in oracle_run_steps.  Returns the number of penetrating nodes. */

@@ -56,6 +56,9 @@ def load() -> C.CDLL:
         lib.oracle_run_lanes.argtypes = [C.POINTER(_abi.ModelView), C.c_int32, C.POINTER(_abi.Results)]
         lib.oracle_run_steps_lanes.argtypes = [C.POINTER(_abi.ModelView), C.c_int32, C.c_int32, f64p, f64p, i32p,
                                                u8p, f64p, i32p]
+        lib.oracle_run_displacements.argtypes = [C.POINTER(_abi.ModelView), C.c_int32, C.c_int32, f64p, i32p]
+        lib.oracle_set_element_hook.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
+        lib.oracle_set_element_hook.restype = None
         lib.lane_blocks_create.argtypes = [C.c_int]
         lib.lane_blocks_create.restype = C.c_void_p
         lib.lane_blocks_free.argtypes = [C.c_void_p]

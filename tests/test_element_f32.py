@@ -86,3 +86,33 @@ def test_honeycomb_f32_tracks_f64_under_crush_with_shear():
             assert np.all(np.abs(st32[18:24] - st64[18:24]) < floor), step
             assert np.all(np.abs(tau32 - tau64.ravel()) < floor * 1.0), step
         assert st64[24] > 0.3
+
+
+@pytest.mark.parametrize("case", ["crash", "toml"])
+def test_product_element_math_inside_oracle_solve(case):
+    """The GPU kernels' f32 element force (honeycomb and Hencky, relative
+    displacements, packed hourglass H; compiled for the host) substituted
+    for the oracle's element force in the oracle's own solve loop: whole
+    solves stay within the parity bars (delta-v 0.05 m/s, crush 2 mm)."""
+    import crushrs_amd as ca
+    if case == "crash":
+        make = lambda: ca.Model.crash(30.0, 30.0)
+    else:
+        a = ca.renamed(ca.vehicle_file("neon_1996"), "neon_a")
+        b = ca.renamed(ca.vehicle_file("expedition_1999"), "expedition_b")
+        make = lambda: ca.Model.headon(a, b, 25 * ca.MPH, 20 * ca.MPH, offset=0.3, end_time=0.08)
+    o = of.run(make())
+    m = make()
+    ctx, fn = hk.element_hook(m)
+    L = of.load()
+    L.oracle_set_element_hook(fn, ctx, 3)
+    try:
+        r = of.run(m)
+    finally:
+        L.oracle_set_element_hook(None, None, 0)
+        hk.load().hk_hook_free(ctx)
+    assert r.eroded_elements == o.eroded_elements
+    assert abs(r.steps - o.steps) <= max(1, o.steps // 100)
+    for x, y in zip(r.groups, o.groups):
+        assert np.all(np.abs(x.delta_v - y.delta_v) < 0.05), (x.delta_v, y.delta_v)
+        assert abs(x.crush_max - y.crush_max) < 2e-3
