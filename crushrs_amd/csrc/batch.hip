@@ -157,6 +157,7 @@ ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     o.g_mass = b.put(P.g_mass);
     o.g_v0 = b.put(P.g_v0);
     o.g_dv = b.zeros(sizeof(double) * 3 * std::max(P.n_groups, 1));
+    o.cgp = b.zeros(sizeof(FaceGrid) * std::max<size_t>(P.cdesc.size(), 1));
     {
         StepCtl c = initial_ctl(P);
         o.ctl = b.put_raw(&c, sizeof c);
@@ -176,7 +177,6 @@ ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     o.fh_list = scratch(sizeof(int32_t) * FACE_HITS_CAP * (size_t)P.n_cfaces);
     o.face_force = scratch(sizeof(double) * 3 * (size_t)P.n_cfaces);
     o.cgrid = scratch(sizeof(int32_t) * grid_ints);
-    o.cgp = scratch(sizeof(FaceGrid) * P.cdesc.size());
     o.dtp_min = scratch(sizeof(double) * dtp_slots(P));
     o.dtp_sum = scratch(sizeof(double) * dtp_slots(P));
     o.total = off;
@@ -383,21 +383,31 @@ __global__ __launch_bounds__(B) void k_grid(const InstDev* __restrict__ table, c
     const ContactDesc cd = D.cdesc[ci];
     for (int f = threadIdx.x; f < cd.n_faces; f += B) contact_face(D, cd.face_begin + f);
     __syncthreads();
-    FaceGrid g = build_face_grid<B>(D, cd, cell_scale, BATCH_GRID_CELLS, BATCH_GRID_ENTRIES, s_start, s_cursor,
-                                    s_entries, s_w, s_tmp);
     int32_t* gs = D.cgrid + cd.grid_off;
     int32_t* ge = gs + cd.grid_cells + 1;
-    if (!g.overflow) {
-        const int ncell = g.dim[0] * g.dim[1] * g.dim[2];
-        for (int i = threadIdx.x; i <= ncell; i += B) gs[i] = s_start[i];
-        const int total = s_start[ncell];
-        for (int i = threadIdx.x; i < total; i += B) ge[i] = s_entries[i];
-    } else if (cd.n_faces > BATCH_GRID_LARGE) {
-        // too many entries for LDS: build the grid again in this contact's
-        // device-memory region, whose capacities grow with its face count
+    const bool big = cd.n_faces > BATCH_GRID_LARGE;
+    // a contact whose grid overflowed LDS at the last step is built in its
+    // device-memory region directly (the grid layout never changes which
+    // face a node hits: the search keeps the lowest accepting index)
+    const bool dev_first = big && D.cgp[ci].in_dev;
+    FaceGrid g;
+    if (!dev_first) {
+        g = build_face_grid<B>(D, cd, cell_scale, BATCH_GRID_CELLS, BATCH_GRID_ENTRIES, s_start, s_cursor, s_entries,
+                               s_w, s_tmp);
+        if (!g.overflow) {
+            const int ncell = g.dim[0] * g.dim[1] * g.dim[2];
+            for (int i = threadIdx.x; i <= ncell; i += B) gs[i] = s_start[i];
+            const int total = s_start[ncell];
+            for (int i = threadIdx.x; i < total; i += B) ge[i] = s_entries[i];
+        }
+    }
+    if (dev_first || (g.overflow && big)) {
+        // too many entries for LDS: the grid in this contact's device-memory
+        // region, whose capacities grow with its face count
         __syncthreads();
         g = build_face_grid<B>(D, cd, cell_scale, cd.grid_cells, cd.grid_entries, gs, ge + cd.grid_entries, ge, s_w,
                                s_tmp);
+        g.in_dev = 1;
     }
     if (threadIdx.x == 0) D.cgp[ci] = g;
 }

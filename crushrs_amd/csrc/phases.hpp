@@ -163,6 +163,8 @@ struct FaceGrid {
     double h;
     int dim[3];
     int overflow;
+    int in_dev;  // batched path: built in device memory (the LDS grid overflowed)
+    int reserved;
 };
 
 __device__ __forceinline__ int grid_cell(const FaceGrid& g, double x, int d) {
@@ -596,6 +598,8 @@ __device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, dou
         h *= 1.25;
     }
     g.h = h;
+    g.in_dev = 0;
+    g.reserved = 0;
     g.overflow = (nf == 0 || cells > MAXC || !(isfinite(g.glo[0]) && isfinite(ghi[0]))) ? 1 : 0;
     const int dx = g.dim[0], dy = g.dim[1], dz = g.dim[2];
     const int ncell = g.overflow ? 1 : dx * dy * dz;
