@@ -192,6 +192,7 @@ class GroupResult:
     delta_v: np.ndarray
     crush_end: float
     crush_max: float
+    pulse_delta_v: float = 0.0
 
 
 @dataclass
@@ -208,6 +209,10 @@ class Results:
     velocity: np.ndarray
     masses: np.ndarray
     groups: list = field(default_factory=list)
+    # accelerometer pulses [n_groups][samples], unfiltered local-x m/s^2 at
+    # t = k * pulse_dt (None when the model records none)
+    pulse: Optional[np.ndarray] = None
+    pulse_dt: float = 0.0
 
     def mean_velocity(self, nodes) -> tuple[float, np.ndarray]:
         """Results::mean_velocity / Model::mean_velocity (src/model.rs:281-295)."""
@@ -219,7 +224,10 @@ class Results:
         return float(mt), (m[:, None] * self.velocity[nodes]).sum(0) / mt
 
 
-def alloc_results(n_nodes: int, n_groups: int):
+PULSE_CAPACITY = 8192  # samples per group kept in Results.pulse (0.8 s at 1e-4 s)
+
+
+def alloc_results(n_nodes: int, n_groups: int, pulse_capacity: int = PULSE_CAPACITY):
     u = np.zeros((n_nodes, 3))
     v = np.zeros((n_nodes, 3))
     m = np.zeros(n_nodes)
@@ -229,14 +237,23 @@ def alloc_results(n_nodes: int, n_groups: int):
     r.velocity = v.ctypes.data_as(_abi.f64p)
     r.masses = m.ctypes.data_as(_abi.f64p)
     r.groups = C.cast(g, C.POINTER(_abi.GroupResult))
+    p = np.zeros((max(min(n_groups, 8), 1), pulse_capacity))
+    r.pulse = p.ctypes.data_as(_abi.f64p)
+    r.pulse_capacity = pulse_capacity
+    r.pulse_buffer = p  # keeps the array alive with the struct
     return r, u, v, m, g
 
 
-def to_results(r, u, v, m, g) -> Results:
-    groups = [GroupResult(g[i].mass, np.array(g[i].delta_v[:]), g[i].crush_end, g[i].crush_max)
+def to_results(r, u, v, m, g, pulse_dt: float = 0.0) -> Results:
+    groups = [GroupResult(g[i].mass, np.array(g[i].delta_v[:]), g[i].crush_end, g[i].crush_max, g[i].pulse_delta_v)
               for i in range(r.n_groups)]
+    pulse = None
+    buf = getattr(r, "pulse_buffer", None)
+    if buf is not None and r.pulse_samples > 0:
+        ns = min(int(r.pulse_samples), r.pulse_capacity)
+        pulse = buf[:min(r.n_groups, 8), :ns].copy()
     return Results(r.time, int(r.steps), int(r.eroded_elements), r.wall_ms, r.device_ms, r.added_mass_max,
-                   bool(r.separated), u, v, m, groups)
+                   bool(r.separated), u, v, m, groups, pulse, pulse_dt)
 
 
 def run(model: Model) -> Results:
@@ -244,7 +261,7 @@ def run(model: Model) -> Results:
     lib = load()
     r, u, v, m, g = alloc_results(model.n_nodes, model.view.n_groups)
     check(lib.crush_run(model.ptr, C.byref(r)))
-    return to_results(r, u, v, m, g)
+    return to_results(r, u, v, m, g, model.settings.pulse_dt)
 
 
 def debug_run_steps(model: Model, n_steps: int):
@@ -322,7 +339,7 @@ class Batch:
             r.displacement = None
             r.velocity = None
         check(self._lib.crush_batch_result(self._h, i, C.byref(r)))
-        return to_results(r, u, v, m, g)
+        return to_results(r, u, v, m, g, self.models[i].settings.pulse_dt)
 
 
 def server_options(**kw) -> _abi.ServerOptions:
@@ -363,16 +380,17 @@ class Server:
     def submit(self, model: Model, fields: bool = False) -> int:
         t = C.c_int64()
         check(self._lib.crush_server_submit(self._h, model.ptr, _abi.SUBMIT_FIELDS if fields else 0, C.byref(t)))
-        self._shape[t.value] = (model.n_nodes, model.view.n_groups, fields)
+        self._shape[t.value] = (model.n_nodes, model.view.n_groups, fields, model.settings.pulse_dt)
         return t.value
 
     def _alloc(self, ticket):
         # an unknown ticket gets empty outputs; the library reports the error
-        n_nodes, n_groups, fields = self._shape.get(ticket, (0, 0, False))
+        n_nodes, n_groups, fields, pulse_dt = self._shape.get(ticket, (0, 0, False, 0.0))
         r, u, v, m, g = alloc_results(n_nodes, n_groups)
         if not fields:
             r.displacement = None
             r.velocity = None
+        r.pulse_dt_value = pulse_dt
         return r, u, v, m, g
 
     def poll(self, ticket: int) -> Optional[Results]:
@@ -382,13 +400,13 @@ class Server:
         if not done.value:
             return None
         self._shape.pop(ticket, None)
-        return to_results(r, u, v, m, g)
+        return to_results(r, u, v, m, g, r.pulse_dt_value)
 
     def wait(self, ticket: int, timeout_ms: float = 0.0) -> Results:
         r, u, v, m, g = self._alloc(ticket)
         check(self._lib.crush_server_wait(self._h, ticket, C.byref(r), timeout_ms))
         self._shape.pop(ticket, None)
-        return to_results(r, u, v, m, g)
+        return to_results(r, u, v, m, g, r.pulse_dt_value)
 
     def stats(self) -> _abi.ServerStats:
         st = _abi.ServerStats()

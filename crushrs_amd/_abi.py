@@ -65,6 +65,7 @@ class Settings(C.Structure):
         ("mass_scaling", C.c_double),
         ("adaptive_check_steps", C.c_int32),
         ("separation_steps", C.c_int32),
+        ("pulse_dt", C.c_double),
     ]
 
 
@@ -76,6 +77,10 @@ class Group(C.Structure):
         ("front", i32p),
         ("accel_node", C.c_int32),
         ("axis", C.c_int32),
+        ("frame_origin", C.c_int32),
+        ("frame_x", C.c_int32),
+        ("heading", C.c_int32),
+        ("reserved0", C.c_int32),
     ]
 
 
@@ -108,6 +113,7 @@ class GroupResult(C.Structure):
         ("delta_v", C.c_double * 3),
         ("crush_end", C.c_double),
         ("crush_max", C.c_double),
+        ("pulse_delta_v", C.c_double),
     ]
 
 
@@ -125,6 +131,9 @@ class Results(C.Structure):
         ("velocity", f64p),
         ("masses", f64p),
         ("groups", C.POINTER(GroupResult)),
+        ("pulse", f64p),
+        ("pulse_capacity", C.c_int32),
+        ("pulse_samples", C.c_int32),
     ]
 
 
@@ -245,6 +254,16 @@ class ServerStats(C.Structure):
 
 SUBMIT_FIELDS = 1
 
+
+class PulseMetrics(C.Structure):
+    _fields_ = [
+        ("t_max_crush", C.c_double),
+        ("max_crush", C.c_double),
+        ("v_end", C.c_double),
+        ("restitution", C.c_double),
+        ("peak_accel", C.c_double),
+    ]
+
 LIB_PATH = Path(__file__).resolve().parent / "libcrushrs_amd.so"
 
 # Every symbol the public headers declare, with (restype, argtypes).
@@ -287,6 +306,12 @@ SIGNATURES = {
     "crush_model_node_set": (C.c_int32, [C.c_void_p, C.c_char_p, C.c_int32, C.POINTER(i32p), i32p]),
     "crush_model_part_names": (C.c_int32, [C.c_void_p, C.c_char_p, C.c_int32]),
     "crush_model_free": (None, [C.c_void_p]),
+    "crush_cfc_filter": (C.c_int32, [f64p, C.c_int32, C.c_double, C.c_double, f64p]),
+    "crush_resample": (C.c_int32, [f64p, f64p, C.c_int32, C.c_double, C.c_double, C.c_double, f64p, C.c_int32,
+                                   i32p]),
+    "crush_integrate": (C.c_int32, [f64p, f64p, C.c_int32, C.c_double, f64p]),
+    "crush_pulse_metrics_get": (C.c_int32, [f64p, f64p, C.c_int32, C.c_double, C.c_double,
+                                            C.POINTER(PulseMetrics)]),
 }
 
 _lib = None

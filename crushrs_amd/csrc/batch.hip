@@ -22,6 +22,7 @@
 // captured in one hipGraph and replayed until every instance has retired;
 // retired instances leave the active lists between replays.
 #include "engine.hpp"
+#include "signal.hpp"
 
 #include <hip/hip_runtime.h>
 
@@ -132,8 +133,8 @@ ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
         cd.grid_entries = BATCH_GRID_ENTRIES;
         const bool big = cd.n_faces > BATCH_GRID_LARGE;
         if (big) {
-            cd.grid_cells = std::min(std::max(8 * cd.n_faces, BATCH_GRID_CELLS), 1 << 16);
-            cd.grid_entries = std::min(std::max(64 * cd.n_faces, BATCH_GRID_ENTRIES), 1 << 21);
+            cd.grid_cells = std::min(std::max(8 * cd.n_faces, BIG_GRID_CELLS), 1 << 16);
+            cd.grid_entries = std::min(std::max(64 * cd.n_faces, BIG_GRID_ENTRIES), 1 << 21);
         }
         cd.grid_off = (int32_t)grid_ints;
         grid_ints += (size_t)cd.grid_cells + 1 + cd.grid_entries + (big ? cd.grid_cells : 0);
@@ -157,6 +158,7 @@ ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     o.g_mass = b.put(P.g_mass);
     o.g_v0 = b.put(P.g_v0);
     o.g_dv = b.zeros(sizeof(double) * 3 * std::max(P.n_groups, 1));
+    o.g_frame = b.put(P.g_frame);
     o.cgp = b.zeros(sizeof(FaceGrid) * std::max<size_t>(P.cdesc.size(), 1));
     {
         StepCtl c = initial_ctl(P);
@@ -179,6 +181,9 @@ ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     o.cgrid = scratch(sizeof(int32_t) * grid_ints);
     o.dtp_min = scratch(sizeof(double) * dtp_slots(P));
     o.dtp_sum = scratch(sizeof(double) * dtp_slots(P));
+    // pulses: filled in order from sample 0 as the steps end, never read
+    // before they are written
+    o.pulse = scratch(sizeof(double) * (size_t)std::min(P.n_groups, 8) * (size_t)P.n_pulse);
     o.total = off;
     out.swap(b.bytes);
     return o;
@@ -246,6 +251,10 @@ InstDev make_inst(const Prepared& P, const ImageOffsets& o, uintptr_t base) {
     I.g_mass = at<double>(base, o.g_mass);
     I.g_v0 = at<double>(base, o.g_v0);
     I.g_dv = at<double>(base, o.g_dv);
+    I.g_frame = at<int32_t>(base, o.g_frame);
+    I.pulse = at<double>(base, o.pulse);
+    I.n_pulse = P.n_pulse;
+    I.pulse_dt = P.settings.pulse_dt;
     I.ctl = at<StepCtl>(base, o.ctl);
     I.cgrid = at<int32_t>(base, o.cgrid);
     I.cgp = at<FaceGrid>(base, o.cgp);
@@ -366,7 +375,52 @@ __global__ __launch_bounds__(B) void k_end(const InstDev* __restrict__ table, co
 
 // Contact (src/contact.rs:92-160), in four launches over all open steps.
 // (1) geometry of the faces of one contact of one instance and its face
-// grid, built in LDS and stored to device memory.
+// grid, built in LDS and stored to device memory: a 1024-cell grid
+// (k_grid, every contact); for a contact of more than BATCH_GRID_LARGE
+// faces whose grid overflowed it, a 2048-cell grid in 64 KB of LDS
+// (k_grid_big, launched after k_grid in the same step); when that overflows
+// too, a grid built in the contact's device-memory region.  A contact stays
+// at the level it reached.  The grid layout never changes which face a node
+// hits: the search keeps the lowest accepting face index.
+template <int B, int MAXC, int MAXE, bool BIG>
+__device__ __forceinline__ void grid_contact(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
+                                             const int32_t* __restrict__ n_tiles, double cell_scale, int* s_start,
+                                             int* s_cursor, int* s_entries, double* s_w, int* s_tmp) {
+    if ((int)blockIdx.x >= *n_tiles) return;
+    const int2 tile = tiles[blockIdx.x];
+    const InstDev& D = table[tile.x];
+    if (!D.ctl->pending) return;
+    const int ci = tile.y;
+    const ContactDesc cd = D.cdesc[ci];
+    const bool can_grow = cd.n_faces > BATCH_GRID_LARGE;
+    const int mode = can_grow ? D.cgp[ci].mode : 0;
+    if (BIG ? mode == 0 : mode != 0) return;  // the other kernel builds this grid
+    for (int f = threadIdx.x; f < cd.n_faces; f += B) contact_face(D, cd.face_begin + f);
+    __syncthreads();
+    int32_t* gs = D.cgrid + cd.grid_off;
+    int32_t* ge = gs + cd.grid_cells + 1;
+    FaceGrid g;
+    int next = mode;
+    if (mode < 2) {
+        g = build_face_grid<B>(D, cd, cell_scale, MAXC, MAXE, s_start, s_cursor, s_entries, s_w, s_tmp);
+        if (!g.overflow) {
+            const int ncell = g.dim[0] * g.dim[1] * g.dim[2];
+            for (int i = threadIdx.x; i <= ncell; i += B) gs[i] = s_start[i];
+            const int total = s_start[ncell];
+            for (int i = threadIdx.x; i < total; i += B) ge[i] = s_entries[i];
+        } else if (can_grow) {
+            next = mode + 1;
+        }
+    }
+    if (BIG && next == 2) {
+        __syncthreads();
+        g = build_face_grid<B>(D, cd, cell_scale, cd.grid_cells, cd.grid_entries, gs, ge + cd.grid_entries, ge, s_w,
+                               s_tmp);
+    }
+    g.mode = next;
+    if (threadIdx.x == 0) D.cgp[ci] = g;
+}
+
 template <int B>
 __global__ __launch_bounds__(B) void k_grid(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
                                             const int32_t* __restrict__ n_tiles, double cell_scale) {
@@ -375,41 +429,20 @@ __global__ __launch_bounds__(B) void k_grid(const InstDev* __restrict__ table, c
     __shared__ int s_entries[BATCH_GRID_ENTRIES];
     __shared__ double s_w[B / 64];
     __shared__ int s_tmp[B / 64];
-    if ((int)blockIdx.x >= *n_tiles) return;
-    const int2 tile = tiles[blockIdx.x];
-    const InstDev& D = table[tile.x];
-    if (!D.ctl->pending) return;
-    const int ci = tile.y;
-    const ContactDesc cd = D.cdesc[ci];
-    for (int f = threadIdx.x; f < cd.n_faces; f += B) contact_face(D, cd.face_begin + f);
-    __syncthreads();
-    int32_t* gs = D.cgrid + cd.grid_off;
-    int32_t* ge = gs + cd.grid_cells + 1;
-    const bool big = cd.n_faces > BATCH_GRID_LARGE;
-    // a contact whose grid overflowed LDS at the last step is built in its
-    // device-memory region directly (the grid layout never changes which
-    // face a node hits: the search keeps the lowest accepting index)
-    const bool dev_first = big && D.cgp[ci].in_dev;
-    FaceGrid g;
-    if (!dev_first) {
-        g = build_face_grid<B>(D, cd, cell_scale, BATCH_GRID_CELLS, BATCH_GRID_ENTRIES, s_start, s_cursor, s_entries,
-                               s_w, s_tmp);
-        if (!g.overflow) {
-            const int ncell = g.dim[0] * g.dim[1] * g.dim[2];
-            for (int i = threadIdx.x; i <= ncell; i += B) gs[i] = s_start[i];
-            const int total = s_start[ncell];
-            for (int i = threadIdx.x; i < total; i += B) ge[i] = s_entries[i];
-        }
-    }
-    if (dev_first || (g.overflow && big)) {
-        // too many entries for LDS: the grid in this contact's device-memory
-        // region, whose capacities grow with its face count
-        __syncthreads();
-        g = build_face_grid<B>(D, cd, cell_scale, cd.grid_cells, cd.grid_entries, gs, ge + cd.grid_entries, ge, s_w,
-                               s_tmp);
-        g.in_dev = 1;
-    }
-    if (threadIdx.x == 0) D.cgp[ci] = g;
+    grid_contact<B, BATCH_GRID_CELLS, BATCH_GRID_ENTRIES, false>(table, tiles, n_tiles, cell_scale, s_start, s_cursor,
+                                                                 s_entries, s_w, s_tmp);
+}
+
+template <int B>
+__global__ __launch_bounds__(B) void k_grid_big(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
+                                                const int32_t* __restrict__ n_tiles, double cell_scale) {
+    __shared__ int s_start[BIG_GRID_CELLS + 1];
+    __shared__ int s_cursor[BIG_GRID_CELLS];
+    __shared__ int s_entries[BIG_GRID_ENTRIES];
+    __shared__ double s_w[B / 64];
+    __shared__ int s_tmp[B / 64];
+    grid_contact<B, BIG_GRID_CELLS, BIG_GRID_ENTRIES, true>(table, tiles, n_tiles, cell_scale, s_start, s_cursor,
+                                                            s_entries, s_w, s_tmp);
 }
 
 // (2) node search, tiles of T contacting nodes of one contact of one
@@ -563,6 +596,7 @@ __global__ void k_collect(const InstDev* __restrict__ table, int n, BatchSummary
     r.done = c.done;
     r.error = c.error;
     r.separated = c.separated;
+    r.pulse_samples = c.pulse_k;
     r.t_retire = c.t_retire;
     for (int g = 0; g < 8; ++g) {
         r.crush_max[g] = g < D.n_groups ? c.crush_max[g] : 0.0;
@@ -599,6 +633,7 @@ static void add_tiles(const Prepared& P, int i, int T, std::vector<int2>* out) {
         const ContactDesc& cd = P.cdesc[ci];
         if (cd.n_faces == 0) continue;
         out[L_GRID].push_back(make_int2(i, ci));
+        if (cd.n_faces > BATCH_GRID_LARGE) out[L_GRIDB].push_back(make_int2(i, ci));
         for (int q = 0; q < cd.n_nodes; q += CONTACT_TILE) out[L_SEARCH].push_back(make_int2(i, cd.node_begin + q));
     }
     for (int k = 0; k < (int)P.tn_node.size(); k += CONTACT_TILE) out[L_APPLY].push_back(make_int2(i, k));
@@ -654,6 +689,9 @@ static void launch_step(BatchImpl& bi, double cs, hipEvent_t* ev) {
     mark(K_GRID);
     if (bi.cap[L_GRID] > 0)
         hipLaunchKernelGGL(k_grid<B>, dim3(bi.cap[L_GRID]), dim3(B), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID, cs);
+    if (bi.cap[L_GRIDB] > 0)
+        hipLaunchKernelGGL(k_grid_big<B>, dim3(bi.cap[L_GRIDB]), dim3(B), 0, st, tab, bi.d_tiles[L_GRIDB].p,
+                           cnt + L_GRIDB, cs);
     mark(K_SEARCH);
     if (bi.cap[L_SEARCH] > 0)
         hipLaunchKernelGGL(k_search_tiles<CONTACT_TILE>, dim3(bi.cap[L_SEARCH]), dim3(CONTACT_TILE), 0, st, tab,
@@ -1079,6 +1117,7 @@ void batch_result(BatchImpl& bi, int i, crush_results* out) {
             gr.crush_end = g < 8 ? r.crush_end[g] : 0.0;
             gr.crush_max = g < 8 ? r.crush_max[g] : 0.0;
         }
+    pulse_results(P, reinterpret_cast<const double*>(base + o.pulse), r.pulse_samples, out);
 }
 
 void batch_free(BatchImpl* bi) { delete bi; }

@@ -14,7 +14,7 @@ struct BatchSummary {
     long long steps;
     long long eroded;
     double added_max;
-    int done, error, separated, reserved;
+    int done, error, separated, pulse_samples;
     double dv[8][3];
     double crush_end[8];
     double crush_max[8];

@@ -43,6 +43,12 @@ struct StepCtl {
     double crush_end[8];
     unsigned long long t_retire;  // device wall clock at retirement (batched path)
     unsigned long long reserved2;
+    // accelerometer pulse (groups < 8): the last sample's time and local-x
+    // accelerations, and the next grid index k of t = k pulse_dt to fill
+    double pulse_t;
+    double pulse_a[8];
+    int pulse_k;
+    int reserved4;
 };
 
 // Face geometry record width (doubles) in the SoA face array:

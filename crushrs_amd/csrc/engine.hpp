@@ -16,9 +16,13 @@ namespace crush {
 
 constexpr int BATCH_GRID_CELLS = 1024;
 constexpr int BATCH_GRID_ENTRIES = 4096;
-// contacts with more faces than this get a device-memory grid region sized
-// to their face count (used when the LDS grid overflows)
-constexpr int BATCH_GRID_LARGE = 128;
+// Contacts with more faces than this may outgrow the 1024-cell LDS grid:
+// their grids then move to a kernel with a 64 KB LDS grid (BIG_GRID_CELLS
+// cells, BIG_GRID_ENTRIES entries), and past that to a device-memory region
+// of 8 cells and 64 entries per face (at least the large LDS capacities).
+constexpr int BATCH_GRID_LARGE = 512;
+constexpr int BIG_GRID_CELLS = 2048;
+constexpr int BIG_GRID_ENTRIES = 12000;
 constexpr int NODE_TILE = 256;
 constexpr int CONTACT_TILE = 64;
 
@@ -28,14 +32,14 @@ struct ImageOffsets {
     size_t conn, ecls, cls_mat, cgrad, cvol, cgam, chs, cdtf, emass, mats, state, eroded, extra;
     size_t X, u, v, f_int, f_ext, inv_mass, mass, added, fixed, n2e_ptr, n2e_idx;
     size_t cdesc, cn_node, cn_scale, cf_nodes, cf_contact, tn_node, tn_ptr, tn_ent, rec_face, fh_count;
-    size_t g_nodes_ptr, g_nodes, g_front_ptr, g_front, g_accel, g_axis, g_mass, g_v0, g_dv, ctl;
+    size_t g_nodes_ptr, g_nodes, g_front_ptr, g_front, g_accel, g_axis, g_mass, g_v0, g_dv, g_frame, ctl;
     size_t init_bytes;
-    size_t fe, fgeo, rec_f, fh_list, face_force, cgrid, cgp, dtp_min, dtp_sum;
+    size_t fe, fgeo, rec_f, fh_list, face_force, cgrid, cgp, dtp_min, dtp_sum, pulse;
     size_t total;
 };
 
 // Active lists: the instance list and the tile lists of the tile kernels.
-enum { L_INST, L_HC, L_EL, L_NODE, L_FACE, L_GRID, L_SEARCH, L_APPLY, N_LISTS };
+enum { L_INST, L_HC, L_EL, L_NODE, L_FACE, L_GRID, L_SEARCH, L_APPLY, L_GRIDB, N_LISTS };
 // Kernels of one step, in launch order.
 enum { K_END, K_GRID, K_SEARCH, K_FACEFORCE, K_APPLY, K_HALF, K_FORCE_HC, K_FORCE_EL, K_GATHER, N_KERNELS };
 

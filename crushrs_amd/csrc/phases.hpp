@@ -91,6 +91,11 @@ struct InstDev {
     const double* g_mass;
     const double* g_v0;   // [3G] initial mean velocities
     double* g_dv;         // [3G] delta-v written when the solve ends
+    const int32_t* g_frame;  // [2G] accelerometer frame nodes (origin, x), -1 = global x
+    // accelerometer pulses [min(G, 8)][n_pulse] at t = k pulse_dt (n_pulse 0: none)
+    double* pulse;
+    int n_pulse;
+    double pulse_dt;
     StepCtl* ctl;
     // batched path: per-tile partials of the stable step (min dt_e, sum of
     // scaling masses), one slot per force tile of the instance
@@ -163,7 +168,10 @@ struct FaceGrid {
     double h;
     int dim[3];
     int overflow;
-    int in_dev;  // batched path: built in device memory (the LDS grid overflowed)
+    // batched path, contacts with more than BATCH_GRID_LARGE faces: where the
+    // grid is built, 0 small LDS grid, 1 large LDS grid, 2 device memory;
+    // raised when a grid overflows and kept at later steps
+    int mode;
     int reserved;
 };
 
@@ -585,20 +593,42 @@ __device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, dou
     }
     sz = block_reduce<B>(sz, 2, s_w);
     double h = fmax(sz * cell_scale, 1e-9);
+    // cell size: grown until the grid has at most MAXC cells, then doubled
+    // (at most 4 times) while the faces' candidate boxes would make more
+    // than MAXE entries
     long long cells = 1;
-    for (int iter = 0; iter < 64; ++iter) {
-        cells = 1;
+    for (int attempt = 0;; ++attempt) {
+        for (int iter = 0; iter < 64; ++iter) {
+            cells = 1;
 #pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const double ext = ghi[d] - g.glo[d];
-            g.dim[d] = ext > 0.0 ? max((int)fmin(ceil(ext / h), 1e6), 1) : 1;
-            cells *= g.dim[d];
+            for (int d = 0; d < 3; ++d) {
+                const double ext = ghi[d] - g.glo[d];
+                g.dim[d] = ext > 0.0 ? max((int)fmin(ceil(ext / h), 1e6), 1) : 1;
+                cells *= g.dim[d];
+            }
+            if (cells <= MAXC) break;
+            h *= 1.25;
         }
-        if (cells <= MAXC) break;
-        h *= 1.25;
+        g.h = h;
+        if (attempt == 4 || cells > MAXC || nf == 0 || !(isfinite(g.glo[0]) && isfinite(ghi[0]))) break;
+        double entries = 0.0;
+        for (int f = t; f < nf; f += B) {
+            double flo[3], fhi[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                flo[d] = D.fgeo[(size_t)(FG_CLO + d) * F + fb + f];
+                fhi[d] = D.fgeo[(size_t)(FG_CHI + d) * F + fb + f];
+            }
+            if (!(flo[0] <= fhi[0] && flo[1] <= fhi[1] && flo[2] <= fhi[2])) continue;
+            entries += (double)(grid_cell(g, fhi[0], 0) - grid_cell(g, flo[0], 0) + 1) *
+                       (double)(grid_cell(g, fhi[1], 1) - grid_cell(g, flo[1], 1) + 1) *
+                       (double)(grid_cell(g, fhi[2], 2) - grid_cell(g, flo[2], 2) + 1);
+        }
+        entries = block_reduce<B>(entries, 0, s_w);
+        if (entries <= (double)MAXE) break;
+        h *= 2.0;
     }
-    g.h = h;
-    g.in_dev = 0;
+    g.mode = 0;
     g.reserved = 0;
     g.overflow = (nf == 0 || cells > MAXC || !(isfinite(g.glo[0]) && isfinite(ghi[0]))) ? 1 : 0;
     const int dx = g.dim[0], dy = g.dim[1], dz = g.dim[2];
@@ -678,6 +708,52 @@ __device__ void group_means(const InstDev& D, int g, double* s_w, double* out) {
     for (int k = 0; k < 7; ++k) out[k] = block_reduce<B>(s[k], 0, s_w);
 }
 
+// Local-x acceleration of group g's accelerometer node after a step
+// (src/solver.rs:150-187 with the acceleration of :333-342): a = (f_ext -
+// f_int) / m at the node (0 for fixed nodes), projected on the unit vector
+// from the frame's origin node to its x node in the current configuration
+// (src/model.rs:45-61).
+__device__ __forceinline__ double accel_local_x(const InstDev& D, int g) {
+#pragma clang fp contract(off)
+    const int n = D.g_accel[g];
+    if (n < 0 || D.fixed[n]) return 0.0;
+    const double im = D.inv_mass[n];
+    double a[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) a[d] = (D.f_ext[3 * (size_t)n + d] - D.f_int[3 * (size_t)n + d]) * im;
+    const int o = D.g_frame[2 * g], x = D.g_frame[2 * g + 1];
+    if (o < 0) return a[0];
+    double e[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d)
+        e[d] = (D.X[3 * (size_t)x + d] + D.u[3 * (size_t)x + d]) - (D.X[3 * (size_t)o + d] + D.u[3 * (size_t)o + d]);
+    const double en = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    if (en < 1e-300) return a[0];
+    return (e[0] / en) * a[0] + (e[1] / en) * a[1] + (e[2] / en) * a[2];
+}
+
+// Streams the sample (tn, a_g) of every group into the pulses: each grid
+// time k pulse_dt in (t_prev, tn] gets the linear interpolation between the
+// previous sample and this one (the resampling of src/signal.rs:43-64 done
+// as the samples arrive).  Thread 0 of the instance's block only.
+__device__ __forceinline__ void pulse_sample(const InstDev& D, StepCtl* c, double tn) {
+#pragma clang fp contract(off)
+    const int ng = D.n_groups < 8 ? D.n_groups : 8;
+    double a[8];
+    for (int g = 0; g < ng; ++g) a[g] = accel_local_x(D, g);
+    const double tp = c->pulse_t;
+    int k = c->pulse_k;
+    for (; k < D.n_pulse; ++k) {
+        const double tk = (double)k * D.pulse_dt;
+        if (!(tk <= tn)) break;
+        const double w = (tk - tp) / (tn - tp);
+        for (int g = 0; g < ng; ++g) D.pulse[(size_t)g * D.n_pulse + k] = c->pulse_a[g] + w * (a[g] - c->pulse_a[g]);
+    }
+    c->pulse_k = k;
+    c->pulse_t = tn;
+    for (int g = 0; g < ng; ++g) c->pulse_a[g] = a[g];
+}
+
 // End of a step: time, step count, crush samples, separation and the stop
 // decision (the loop condition of src/solver.rs:287 plus the optional
 // retirement at separation).  c is the block's control block in LDS; only
@@ -703,6 +779,7 @@ __device__ bool step_end(const InstDev& D, StepCtl* c, double dt, int pair_hits,
     const bool done = sep || bad || !(tn < c->end_time);
     __syncthreads();
     if (threadIdx.x == 0) {
+        if (D.n_pulse > 0) pulse_sample(D, c, tn);
         for (int g = 0; g < ng; ++g) c->crush_max[g] = fmax(c->crush_max[g], cr[g]);
         c->had_pair = had_pair;
         c->quiet = quiet;

@@ -577,12 +577,28 @@ void prepare(const crush_model& m, Prepared& p) {
             p.g_front.push_back(n);
         }
         if (gr.accel_node >= N || gr.axis < 0 || gr.axis > 2) fail(CRUSH_E_INPUT, "group %d: bad accelerometer", g);
+        if (gr.frame_origin >= N || gr.frame_x >= N || ((gr.frame_origin < 0) != (gr.frame_x < 0)) ||
+            (gr.frame_origin >= 0 && gr.frame_origin == gr.frame_x))
+            fail(CRUSH_E_INPUT, "group %d: bad accelerometer frame", g);
+        p.g_frame.push_back(gr.frame_origin < 0 ? -1 : gr.frame_origin);
+        p.g_frame.push_back(gr.frame_x < 0 ? -1 : gr.frame_x);
+        p.g_heading.push_back(gr.heading > 0 ? 1 : (gr.heading < 0 ? -1 : 0));
         p.g_nodes_ptr.push_back((int32_t)p.g_nodes.size());
         p.g_front_ptr.push_back((int32_t)p.g_front.size());
         p.g_accel.push_back(gr.accel_node);
         p.g_axis.push_back(gr.axis);
         p.g_mass.push_back(mt);
         for (int d = 0; d < 3; ++d) p.g_v0.push_back(mt > 0 ? pv[d] / mt : 0.0);
+    }
+    // pulse grid: t = k pulse_dt up to the end time (the last sample lies at
+    // or before end_time; one slot of slack for its rounding)
+    p.n_pulse = 0;
+    if (s.pulse_dt > 0.0 && m.n_groups > 0) {
+        const double n = std::floor(s.end_time / s.pulse_dt) + 2.0;
+        if (!(n <= 1e7)) fail(CRUSH_E_INPUT, "pulse_dt %g gives too many samples for end_time %g", s.pulse_dt, s.end_time);
+        p.n_pulse = (int)n;
+    } else if (s.pulse_dt < 0.0 || !(s.pulse_dt == s.pulse_dt)) {
+        fail(CRUSH_E_INPUT, "settings: pulse_dt must be >= 0");
     }
 }
 

@@ -74,6 +74,12 @@ struct Prepared {
     // groups (outputs)
     int n_groups = 0;
     std::vector<int32_t> g_nodes_ptr, g_nodes, g_front_ptr, g_front, g_accel, g_axis;
+    // accelerometer frames [2*G] (origin, x-axis node; -1 = global x) and
+    // directions of travel [G]
+    std::vector<int32_t> g_frame, g_heading;
+    // accelerometer pulse samples per group (0 = no pulse): t = k pulse_dt
+    // for k < n_pulse covers [0, end_time]
+    int n_pulse = 0;
     // total mass and lumped masses of each group (for outputs)
     std::vector<double> g_mass, g_v0;  // g_v0: [3*G] initial mean velocity
 };

@@ -191,6 +191,51 @@ int Mesh::merge_coincident_nodes(double tol) {
     return merged;
 }
 
+// Mesh::min_edge_at (src/mesh.rs:237-250): shortest hexahedron edge at a
+// node.
+double Mesh::min_edge_at(int node) const {
+    static const int EDGES[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6},
+                                     {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+    double mn = INFINITY;
+    for (const auto& h : hexes)
+        for (const auto& e : EDGES)
+            if (h[e[0]] == node || h[e[1]] == node) {
+                const Vec3& p = nodes[h[e[0]]];
+                const Vec3& q = nodes[h[e[1]]];
+                mn = std::fmin(mn, std::sqrt((p[0] - q[0]) * (p[0] - q[0]) + (p[1] - q[1]) * (p[1] - q[1]) +
+                                             (p[2] - q[2]) * (p[2] - q[2])));
+            }
+    return mn;
+}
+
+// Mesh::best_aligned_neighbour (src/mesh.rs:262-288): among the part's
+// nodes within `radius` of `node` (excluding `exclude`), the first whose
+// direction from `node` has the largest cosine with `dir`.
+std::pair<int, double> Mesh::best_aligned_neighbour(int node, Vec3 dir, double radius, int part,
+                                                    const std::vector<int>& exclude) const {
+    std::vector<int> cand;
+    if (part >= 0)
+        cand = part_nodes(part);
+    else
+        for (int i = 0; i < (int)nodes.size(); ++i) cand.push_back(i);
+    const Vec3& o = nodes[node];
+    const double dn = std::sqrt(dir[0] * dir[0] + dir[1] * dir[1] + dir[2] * dir[2]);
+    int best = -1;
+    double bc = 0.0;
+    for (int n : cand) {
+        if (n == node || std::find(exclude.begin(), exclude.end(), n) != exclude.end()) continue;
+        const double r[3] = {nodes[n][0] - o[0], nodes[n][1] - o[1], nodes[n][2] - o[2]};
+        const double rn = std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+        if (rn > radius || rn == 0.0) continue;
+        const double c = (r[0] * dir[0] + r[1] * dir[1] + r[2] * dir[2]) / (rn * dn);
+        if (best < 0 || c > bc) {
+            best = n;
+            bc = c;
+        }
+    }
+    return {best, bc};
+}
+
 // src/mesh.rs:549-556: first node of minimum distance among the part's
 // sorted nodes.
 int Mesh::nearest_node(Vec3 at, int part) const {
@@ -289,6 +334,9 @@ crush_settings default_settings() {
     s.velocity_damping = 1.0;
     s.mass_scaling = 0.25;
     s.separation_steps = 0;
+    // the accelerometer pulse at PULSE_DT (src/vehicle.rs:1102), as the
+    // reference's head-on and barrier runs record it (history_steps = 1)
+    s.pulse_dt = 1e-4;
     return s;
 }
 
@@ -397,6 +445,9 @@ void ModelHolder::flatten() {
         cg.front = g.front.data();
         cg.accel_node = g.accel_node;
         cg.axis = g.axis;
+        cg.frame_origin = g.frame_origin;
+        cg.frame_x = g.frame_x;
+        cg.heading = g.heading;
         groups.push_back(cg);
     }
     view = crush_model{};
@@ -966,12 +1017,36 @@ int vehicle_accel_node(const Model& model, const std::string& part, const std::s
     return model.mesh.nearest_node(at, p);
 }
 
-Group make_group(const Model& model, const std::string& name, const std::string& front_face, int axis) {
+// The frame of add_accelerometer_at (src/model.rs:185-213) at node rec of
+// part p: local x from the origin to the x node, which is rec's neighbour
+// best aligned with +X (within 1.75 x the shortest edge at rec, cosine >
+// 0.5), or else rec itself with its best -X neighbour as the origin.  Only
+// local x enters the pulse, so the frame's plane node is not needed.
+static std::pair<int, int> accelerometer_frame(const Mesh& mesh, int rec, int p) {
+    const double radius = 1.75 * mesh.min_edge_at(rec);
+    int origin = rec, x_axis = rec;
+    auto fwd = mesh.best_aligned_neighbour(rec, {1.0, 0.0, 0.0}, radius, p, {});
+    if (fwd.first >= 0 && fwd.second > 0.5) {
+        x_axis = fwd.first;
+    } else {
+        auto back = mesh.best_aligned_neighbour(rec, {-1.0, 0.0, 0.0}, radius, p, {});
+        if (back.first >= 0) origin = back.first;
+    }
+    if (origin == x_axis) fail(CRUSH_E_INPUT, "accelerometer: part '%s' has no neighbouring nodes along X", mesh.parts[p].c_str());
+    return {origin, x_axis};
+}
+
+Group make_group(const Model& model, const std::string& name, const std::string& front_face, int axis,
+                 int heading) {
     Group g;
     if (!model.mesh.nodes_of(name, g.nodes)) fail(CRUSH_E_INPUT, "no node set or part '%s'", name.c_str());
     if (!model.mesh.face_set_nodes(front_face, g.front)) fail(CRUSH_E_INPUT, "no face set '%s'", front_face.c_str());
     g.accel_node = vehicle_accel_node(model, name, front_face);
+    auto fr = accelerometer_frame(model.mesh, g.accel_node, model.mesh.part_index(name));
+    g.frame_origin = fr.first;
+    g.frame_x = fr.second;
     g.axis = axis;
+    g.heading = heading;
     return g;
 }
 
@@ -1005,8 +1080,8 @@ Model build_headon(const crush_vehicle& a, const crush_vehicle& b, double speed_
     double k = std::min(vehicle::contact_stiffness_per_node(a, n_face, h), vehicle::contact_stiffness_per_node(b, n_face, h));
     model.add_contact_pair_scaled(fa, fb, k, 0.3, sa, sb);
     model.settings.end_time = opt.end_time;
-    model.groups.push_back(make_group(model, a.name, fa, 0));
-    model.groups.push_back(make_group(model, b.name, fb, 0));
+    model.groups.push_back(make_group(model, a.name, fa, 0, 1));
+    model.groups.push_back(make_group(model, b.name, fb, 0, -1));
     return model;
 }
 
@@ -1029,8 +1104,8 @@ Model build_crash(double truck_mph, double car_mph) {
                         vehicle::contact_stiffness_per_node(truck, n_face, h));
     model.add_contact_pair_scaled("silverado_front", "neon_front", k, 0.3, {}, {});
     model.settings.end_time = 0.15;
-    model.groups.push_back(make_group(model, "silverado", "silverado_front", 0));
-    model.groups.push_back(make_group(model, "neon", "neon_front", 0));
+    model.groups.push_back(make_group(model, "silverado", "silverado_front", 0, 1));
+    model.groups.push_back(make_group(model, "neon", "neon_front", 0, -1));
     return model;
 }
 
@@ -1069,8 +1144,8 @@ Model build_tbone(const crush_vehicle& truck, const crush_vehicle& car, double s
     std::string tf = std::string(truck.name) + "_front";
     model.add_contact_pair_scaled(tf, side_face, k, 0.3, {}, {});
     model.settings.end_time = 0.15;
-    model.groups.push_back(make_group(model, truck.name, tf, 0));
-    model.groups.push_back(make_group(model, side.name, side_face, 0));
+    model.groups.push_back(make_group(model, truck.name, tf, 0, 1));
+    model.groups.push_back(make_group(model, side.name, side_face, 0, 0));
     return model;
 }
 
@@ -1094,7 +1169,7 @@ Model build_barrier(const crush_vehicle& v, double h, double speed, double end_t
     model.add_contact_pair_scaled(front, "wall_face", vehicle::contact_stiffness_per_node(v, (int)nf.size(), h), 0.3,
                                   scales, {});
     model.settings.end_time = end_time;
-    model.groups.push_back(make_group(model, v.name, front, 0));
+    model.groups.push_back(make_group(model, v.name, front, 0, 1));
     return model;
 }
 

@@ -38,6 +38,10 @@ struct Mesh {
                         const std::vector<std::pair<BlockFace, std::string>>& surfaces);
     int merge_coincident_nodes(double tol);
     int nearest_node(Vec3 at, int part) const;
+    double min_edge_at(int node) const;
+    // (node, cosine), node -1 when none
+    std::pair<int, double> best_aligned_neighbour(int node, Vec3 dir, double radius, int part,
+                                                  const std::vector<int>& exclude) const;
 };
 
 // src/material.rs:41-71,154-237 (Material + Plasticity), flattened.
@@ -64,6 +68,8 @@ struct Group {
     std::vector<int> front;
     int accel_node = -1;
     int axis = 0;
+    int frame_origin = -1, frame_x = -1;  // accelerometer frame (local x from origin to x)
+    int heading = 0;                      // +1 / -1 along global x, 0 unknown
 };
 
 // src/model.rs:152-165 (Model) plus the vehicle groups of the outputs.
@@ -132,7 +138,8 @@ int add_rigid_wall(Mesh& mesh, const std::string& name, double x, std::array<dou
                    std::array<double, 2> z, Heading facing, double thickness);
 // add_vehicle_accelerometer's node (src/vehicle.rs:792-822, model.rs:185-187).
 int vehicle_accel_node(const Model& model, const std::string& part, const std::string& front_face);
-Group make_group(const Model& model, const std::string& name, const std::string& front_face, int axis);
+Group make_group(const Model& model, const std::string& name, const std::string& front_face, int axis,
+                 int heading);
 
 Model build_headon(const crush_vehicle& a, const crush_vehicle& b, double speed_a, double speed_b,
                    const crush_headon_options& opt);

@@ -23,6 +23,7 @@
 
 #include "common.hpp"
 #include "engine.hpp"
+#include "signal.hpp"
 
 namespace crush {
 
@@ -96,7 +97,7 @@ struct Request {
     int status = CRUSH_OK;
     std::string error;
     StepCtl ctl{};
-    std::vector<double> dv, u, v;
+    std::vector<double> dv, u, v, pulse;
 };
 
 }  // namespace
@@ -193,6 +194,7 @@ static void read_result(ServerImpl& S, Request& r) {
         HIP_OK(hipMemcpy(r.u.data(), base + r.o.u, sizeof(double) * r.u.size(), hipMemcpyDeviceToHost));
         HIP_OK(hipMemcpy(r.v.data(), base + r.o.v, sizeof(double) * r.v.size(), hipMemcpyDeviceToHost));
     }
+    r.pulse = pulse_download(P, reinterpret_cast<const double*>(base + r.o.pulse));
     if (r.ctl.error) {
         r.status = CRUSH_E_NONFINITE;
         r.error = "displacement became non-finite at step " + std::to_string(r.ctl.step);
@@ -375,6 +377,7 @@ static void fill(const Request& r, const Prepared& P, crush_results* out) {
             gr.crush_end = g < 8 ? r.ctl.crush_end[g] : 0.0;
             gr.crush_max = g < 8 ? r.ctl.crush_max[g] : 0.0;
         }
+    pulse_results_host(P, r.pulse.empty() ? nullptr : r.pulse.data(), r.ctl.pulse_k, out);
 }
 
 // Hand a finished request to the caller and release its ticket (mu held).

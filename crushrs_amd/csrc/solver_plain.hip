@@ -16,6 +16,7 @@
 #include "hip_util.hpp"
 #include "phases.hpp"
 #include "prep.hpp"
+#include "signal.hpp"
 #include "solver_plain.hpp"
 
 namespace crush {
@@ -34,7 +35,8 @@ struct DeviceModel {
     DevBuf<double> cn_scale, fgeo, rec_f, face_force;
     DevBuf<int32_t> rec_face, fh_count, fh_list;
     DevBuf<int32_t> g_nodes_ptr, g_nodes, g_front_ptr, g_front, g_accel, g_axis;
-    DevBuf<double> g_mass, g_v0, g_dv;
+    DevBuf<double> g_mass, g_v0, g_dv, pulse;
+    DevBuf<int32_t> g_frame;
     DevBuf<StepCtl> ctl;
     InstDev inst{};
 
@@ -106,6 +108,9 @@ struct DeviceModel {
         g_mass.upload(P.g_mass, s);
         g_v0.upload(P.g_v0, s);
         g_dv.alloc(std::max<size_t>(3 * (size_t)P.n_groups, 1));
+        g_frame.upload(P.g_frame, s);
+        // written in order from sample 0 as the steps end
+        pulse.alloc((size_t)std::min(P.n_groups, 8) * (size_t)P.n_pulse);
         StepCtl c;
         std::memset(&c, 0, sizeof c);
         c.t = 0.0;
@@ -182,6 +187,10 @@ struct DeviceModel {
         I.g_mass = g_mass.p;
         I.g_v0 = g_v0.p;
         I.g_dv = g_dv.p;
+        I.g_frame = g_frame.p;
+        I.pulse = pulse.p;
+        I.n_pulse = P.n_pulse;
+        I.pulse_dt = P.settings.pulse_dt;
         I.ctl = ctl.p;
     }
 };
@@ -407,6 +416,7 @@ static void fill_results(const Prepared& P, PlainSolver& S, const StepCtl& c, cr
             r.crush_max = g < 8 ? std::max(c.crush_max[g], cr) : cr;
         }
     }
+    pulse_results(P, S.D->pulse.p, c.pulse_k, out);
 }
 
 // Guard against a solve that never reaches end_time (dt is floored at 1e-15).

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define CRUSH_ABI_VERSION 1
+#define CRUSH_ABI_VERSION 2
 
 /* Status codes.  The reference panics in all of these cases
  * (src/solver.rs:302 non-finite displacement, src/model.rs:186-242 unknown
@@ -97,6 +97,13 @@ typedef struct crush_settings {
      * one before, and (c) groups 0 and 1 move apart:
      * (v1 - v0) . (x1 - x0) > 0 with mass-weighted group means. */
     int32_t separation_steps;
+    /* Accelerometer pulse (src/solver.rs:150-187 sampling, src/signal.rs:
+     * 43-64 resampling): when > 0, the local-x acceleration of every
+     * group's accelerometer node is sampled after each step and resampled
+     * linearly onto t = k * pulse_dt as the solve runs (the reference's
+     * history_steps = 1 with Pulse::from_history at PULSE_DT = 1e-4 s).
+     * 0 = off. */
+    double pulse_dt;
 } crush_settings;
 
 /* A vehicle, for the delta-v / crush outputs of src/bin/crushrs.rs:237-242
@@ -108,6 +115,15 @@ typedef struct crush_group {
     const int32_t* front;      /* nodes of face set <name>_front               */
     int32_t accel_node;        /* rear-seat accelerometer node, -1 = none      */
     int32_t axis;              /* crush axis, 0 = x                            */
+    /* accelerometer frame (src/model.rs:185-213): local x runs from node
+     * frame_origin to node frame_x in the current configuration; -1 = the
+     * global x axis */
+    int32_t frame_origin;
+    int32_t frame_x;
+    /* direction of travel along global x (+1 / -1; 0 = unknown): the pulse
+     * delta-v is reported in that direction (src/headon.rs:70-78) */
+    int32_t heading;
+    int32_t reserved0;
 } crush_group;
 
 /* src/model.rs:152-165 (Model) + src/mesh.rs:37-52 (Mesh), flattened. */
@@ -137,6 +153,11 @@ typedef struct crush_group_result {
     double delta_v[3];     /* final minus initial mass-weighted mean velocity */
     double crush_end;      /* |u_axis(accel) - mean u_axis(front)| at the end */
     double crush_max;      /* max of the same over all steps                  */
+    /* Delta-v from the accelerometer pulse (src/headon.rs:70-78,
+     * tests/impact.rs:305-306): the CFC 60 filtered local-x pulse integrated
+     * over the record, negated for heading +1 (speed at the start minus
+     * speed at the end along the direction of travel).  0 without a pulse. */
+    double pulse_delta_v;
 } crush_group_result;
 
 /* src/solver.rs:63-80 (Results).  Output arrays are caller-owned; pass NULL
@@ -154,6 +175,14 @@ typedef struct crush_results {
     double* velocity;            /* [3*n_nodes] or NULL                        */
     double* masses;              /* [n_nodes] lumped masses, or NULL           */
     crush_group_result* groups;  /* [n_groups] or NULL                         */
+    /* accelerometer pulses, unfiltered local-x acceleration (m/s^2) at
+     * t = k * pulse_dt: pulse[g * pulse_capacity + k] for k < pulse_samples;
+     * NULL for none.  pulse_samples is set whenever pulse_dt > 0 (the count
+     * may exceed pulse_capacity, then only the first pulse_capacity are
+     * written). */
+    double* pulse;
+    int32_t pulse_capacity;
+    int32_t pulse_samples;
 } crush_results;
 
 /* ---- core entry points ------------------------------------------------- */

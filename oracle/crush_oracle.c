@@ -1088,6 +1088,103 @@ static double group_crush(const crush_group* g, const double* u) {
     return fabs(u[3 * g->accel_node + g->axis] - s / g->n_front);
 }
 
+/* ---------------------------------------------------------------- signal */
+
+/* cfc_filter (src/signal.rs:9-39). */
+static void cfc_pass(const double* x, int n, double dt, double cfc, const double* a, const double* b, double* out) {
+    int pad = (int)ceil(3.0 / (cfc * dt));
+    if (pad > n - 1) pad = n - 1;
+    if (pad < 1) pad = 1;
+    int m = n + 2 * pad;
+    double* ext = (double*)malloc(sizeof(double) * (size_t)m);
+    double* y = (double*)malloc(sizeof(double) * (size_t)m);
+    for (int i = 0; i < pad; ++i) ext[i] = 2.0 * x[0] - x[pad - i];
+    for (int i = 0; i < n; ++i) ext[pad + i] = x[i];
+    for (int i = 1; i <= pad; ++i) ext[pad + n - 1 + i] = 2.0 * x[n - 1] - x[n - 1 - i];
+    y[0] = ext[0];
+    y[1] = ext[1];
+    for (int i = 2; i < m; ++i)
+        y[i] = a[0] * ext[i] + a[1] * ext[i - 1] + a[2] * ext[i - 2] + b[0] * y[i - 1] + b[1] * y[i - 2];
+    for (int i = 0; i < n; ++i) out[i] = y[pad + i];
+    free(ext);
+    free(y);
+}
+
+void oracle_cfc_filter(const double* x, int32_t n, double dt, double cfc, double* out) {
+    if (n < 4) {
+        for (int i = 0; i < n; ++i) out[i] = x[i];
+        return;
+    }
+    double wd = 2.0 * 3.141592653589793 * cfc * 2.0775;
+    double wa = sin(wd * dt / 2.0) / cos(wd * dt / 2.0);
+    double s2 = sqrt(2.0);
+    double den = 1.0 + s2 * wa + wa * wa;
+    double a0 = wa * wa / den;
+    double a[3] = {a0, 2.0 * a0, a0};
+    double b[2] = {-2.0 * (wa * wa - 1.0) / den, (-1.0 + s2 * wa - wa * wa) / den};
+    double* f = (double*)malloc(sizeof(double) * (size_t)n);
+    double* r = (double*)malloc(sizeof(double) * (size_t)n);
+    cfc_pass(x, n, dt, cfc, a, b, f);
+    for (int i = 0; i < n; ++i) r[i] = f[n - 1 - i];
+    cfc_pass(r, n, dt, cfc, a, b, f);
+    for (int i = 0; i < n; ++i) out[i] = f[n - 1 - i];
+    free(f);
+    free(r);
+}
+
+/* resample (src/signal.rs:43-64); returns the count, writes at most cap. */
+int32_t oracle_resample(const double* t, const double* y, int32_t n, double t0, double dt, double t_end, double* out,
+                        int32_t cap) {
+    int32_t m = (int32_t)floor((t_end - t0) / dt) + 1;
+    int j = 0;
+    for (int k = 0; k < m && k < cap; ++k) {
+        double tk = t0 + (double)k * dt;
+        while (j + 1 < n && t[j + 1] < tk) ++j;
+        double v;
+        if (j + 1 >= n)
+            v = y[n - 1];
+        else if (tk <= t[0])
+            v = y[0];
+        else {
+            double w = (tk - t[j]) / (t[j + 1] - t[j]);
+            v = y[j] + w * (y[j + 1] - y[j]);
+        }
+        out[k] = v;
+    }
+    return m;
+}
+
+/* integrate (src/signal.rs:67-76). */
+void oracle_integrate(const double* t, const double* y, int32_t n, double y0, double* out) {
+    double acc = y0;
+    if (n <= 0) return;
+    out[0] = acc;
+    for (int i = 1; i < n; ++i) {
+        acc += 0.5 * (y[i] + y[i - 1]) * (t[i] - t[i - 1]);
+        out[i] = acc;
+    }
+}
+
+/* Local-x acceleration of group g's accelerometer node: sample_accelerometers
+ * (src/solver.rs:150-187) with the acceleration of :333-342 and the frame
+ * axis of LocalFrame::axes (src/model.rs:45-61). */
+static double accel_local_x(const crush_model* m, const crush_group* g, const double* u, const double* f_ext,
+                            const double* f_int, const double* inv_mass, const uint8_t* is_fixed) {
+    int n = g->accel_node;
+    if (n < 0) return 0.0;
+    double a[3];
+    for (int d = 0; d < 3; ++d) a[d] = is_fixed[n] ? 0.0 : (f_ext[3 * n + d] - f_int[3 * n + d]) * inv_mass[n];
+    if (g->frame_origin < 0) return a[0];
+    double e[3];
+    for (int d = 0; d < 3; ++d)
+        e[d] = (m->nodes[3 * g->frame_x + d] + u[3 * g->frame_x + d]) -
+               (m->nodes[3 * g->frame_origin + d] + u[3 * g->frame_origin + d]);
+    double en = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    if (en < 1e-300) return a[0];
+    for (int d = 0; d < 3; ++d) e[d] = e[d] / en;
+    return e[0] * a[0] + e[1] * a[1] + e[2] * a[2];
+}
+
 /* solver::run: src/solver.rs:190-381; simd selects the element path
  * (Settings.simd, src/model.rs:100-102). */
 static int solve(const crush_model* m, crush_results* out, taps_t* taps, int simd, int threads) {
@@ -1165,6 +1262,19 @@ static int solve(const crush_model* m, crush_results* out, taps_t* taps, int sim
     compute_forces(m, &el, u, f_int);
     for (int c = 0; c < m->n_contacts; ++c)
         contact_apply_one(m, &m->contacts[c], &rts[c], u, masses, dt, f_ext, NULL);
+    /* accelerometer history (history_steps = 1): t and local-x
+     * acceleration of each group at t = 0 and after every step */
+    uint8_t* is_fixed = (uint8_t*)calloc((size_t)nn + 1, 1);
+    for (int i = 0; i < m->n_fixed; ++i) is_fixed[m->fixed_nodes[i]] = 1;
+    int ng_p = s->pulse_dt > 0.0 ? (m->n_groups < 8 ? m->n_groups : 8) : 0;
+    size_t h_cap = 1024, h_n = 0;
+    double* h_t = (double*)malloc(sizeof(double) * h_cap);
+    double* h_a = (double*)malloc(sizeof(double) * h_cap * 8);
+    if (ng_p > 0) {
+        h_t[0] = 0.0;
+        for (int g = 0; g < ng_p; ++g) h_a[g] = accel_local_x(m, &m->groups[g], u, f_ext, f_int, inv_mass, is_fixed);
+        h_n = 1;
+    }
 
     double t = 0.0;
     long step = 0;
@@ -1252,6 +1362,17 @@ static int solve(const crush_model* m, crush_results* out, taps_t* taps, int sim
         }
         step += 1;
         t += dt;
+        if (ng_p > 0) {
+            if (h_n == h_cap) {
+                h_cap *= 2;
+                h_t = (double*)realloc(h_t, sizeof(double) * h_cap);
+                h_a = (double*)realloc(h_a, sizeof(double) * h_cap * 8);
+            }
+            h_t[h_n] = t;
+            for (int g = 0; g < ng_p; ++g)
+                h_a[8 * h_n + g] = accel_local_x(m, &m->groups[g], u, f_ext, f_int, inv_mass, is_fixed);
+            ++h_n;
+        }
         for (int g = 0; g < m->n_groups && g < 8; ++g)
             crush_max[g] = fmax(crush_max[g], group_crush(&m->groups[g], u));
         /* retirement at separation (crushrs.h, crush_settings) */
@@ -1305,9 +1426,40 @@ done:
                 for (int d = 0; d < 3; ++d) r->delta_v[d] = mt > 0 ? p1[d] / mt - p0[d] / mt : 0.0;
                 r->crush_end = group_crush(gr, u);
                 r->crush_max = g < 8 ? crush_max[g] : r->crush_end;
+                r->pulse_delta_v = 0.0;
             }
         }
+        /* Pulse::from_history + filtered(60) + velocity (src/signal.rs:
+         * 127-146, 184-193; src/headon.rs:70-78) */
+        out->pulse_samples = 0;
+        if (ng_p > 0 && h_n >= 1) {
+            double* ys = (double*)malloc(sizeof(double) * h_n);
+            int32_t ns = (int32_t)floor((h_t[h_n - 1] - 0.0) / s->pulse_dt) + 1;
+            double* pr = (double*)malloc(sizeof(double) * (size_t)ns);
+            double* pf = (double*)malloc(sizeof(double) * (size_t)ns);
+            double* tg = (double*)malloc(sizeof(double) * (size_t)ns);
+            double* vel = (double*)malloc(sizeof(double) * (size_t)ns);
+            for (int k = 0; k < ns; ++k) tg[k] = 0.0 + (double)k * s->pulse_dt;
+            for (int g = 0; g < ng_p; ++g) {
+                for (size_t i = 0; i < h_n; ++i) ys[i] = h_a[8 * i + g];
+                oracle_resample(h_t, ys, (int32_t)h_n, 0.0, s->pulse_dt, h_t[h_n - 1], pr, ns);
+                if (out->pulse && out->pulse_capacity > 0)
+                    for (int k = 0; k < ns && k < out->pulse_capacity; ++k)
+                        out->pulse[(size_t)g * out->pulse_capacity + k] = pr[k];
+                oracle_cfc_filter(pr, ns, s->pulse_dt, 60.0, pf);
+                oracle_integrate(tg, pf, ns, 0.0, vel);
+                if (out->groups) {
+                    int hd = m->groups[g].heading;
+                    out->groups[g].pulse_delta_v = hd < 0 ? vel[ns - 1] : -vel[ns - 1];
+                }
+            }
+            out->pulse_samples = ns;
+            free(ys); free(pr); free(pf); free(tg); free(vel);
+        }
     }
+    free(h_t);
+    free(h_a);
+    free(is_fixed);
     for (int c = 0; c < m->n_contacts; ++c) contact_rt_free(&rts[c]);
     free(rts); free(u); free(v); free(v_half); free(f_int); free(f_ext); free(hits); free(fixed);
     free(masses); free(element_mass); free(added); free(inv_mass); free(dts);

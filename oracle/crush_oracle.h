@@ -102,6 +102,14 @@ int oracle_run_steps_lanes(const crush_model* model, int32_t n_steps, int32_t th
                            double* f_ext, int32_t* hit, uint8_t* eroded, double* dt,
                            int32_t* steps_done);
 
+/* signal.rs restated (src/signal.rs:9-76): the CFC filter (out may not
+ * alias x), linear resampling (returns the sample count, writes at most
+ * cap) and the cumulative trapezoid. */
+void oracle_cfc_filter(const double* x, int32_t n, double dt, double cfc, double* out);
+int32_t oracle_resample(const double* t, const double* y, int32_t n, double t0, double dt, double t_end, double* out,
+                        int32_t cap);
+void oracle_integrate(const double* t, const double* y, int32_t n, double y0, double* out);
+
 /* Test hook: when set, oracle_run / oracle_run_steps (the scalar path)
  * take the force of every element whose material kind k has bit (1 << k)
  * in kinds_mask from fn(ctx, element, kind, u[24] of its nodes, out[24]),

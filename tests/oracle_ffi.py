@@ -57,6 +57,11 @@ def load() -> C.CDLL:
         lib.oracle_run_steps_lanes.argtypes = [C.POINTER(_abi.ModelView), C.c_int32, C.c_int32, f64p, f64p, i32p,
                                                u8p, f64p, i32p]
         lib.oracle_run_displacements.argtypes = [C.POINTER(_abi.ModelView), C.c_int32, C.c_int32, f64p, i32p]
+        lib.oracle_cfc_filter.argtypes = [f64p, C.c_int32, C.c_double, C.c_double, f64p]
+        lib.oracle_cfc_filter.restype = None
+        lib.oracle_resample.argtypes = [f64p, f64p, C.c_int32, C.c_double, C.c_double, C.c_double, f64p, C.c_int32]
+        lib.oracle_integrate.argtypes = [f64p, f64p, C.c_int32, C.c_double, f64p]
+        lib.oracle_integrate.restype = None
         lib.oracle_set_element_hook.argtypes = [C.c_void_p, C.c_void_p, C.c_int32]
         lib.oracle_set_element_hook.restype = None
         lib.lane_blocks_create.argtypes = [C.c_int]
@@ -92,7 +97,7 @@ def run(model):
     r, u, v, m, g = ca.alloc_results(model.n_nodes, model.view.n_groups)
     rc = load().oracle_run(model.ptr, C.byref(r))
     assert rc == 0, f"oracle_run failed: {rc}"
-    return ca.to_results(r, u, v, m, g)
+    return ca.to_results(r, u, v, m, g, model.settings.pulse_dt)
 
 
 def run_lanes(model, threads: int = 1):
@@ -101,7 +106,7 @@ def run_lanes(model, threads: int = 1):
     r, u, v, m, g = ca.alloc_results(model.n_nodes, model.view.n_groups)
     rc = load().oracle_run_lanes(model.ptr, threads, C.byref(r))
     assert rc == 0, f"oracle_run_lanes failed: {rc}"
-    return ca.to_results(r, u, v, m, g)
+    return ca.to_results(r, u, v, m, g, model.settings.pulse_dt)
 
 
 def run_steps(model, n_steps: int, n_contact_nodes: int):

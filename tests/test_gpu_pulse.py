@@ -1,0 +1,96 @@
+"""The accelerometer pulse recorded on the GPU (crush_settings.pulse_dt,
+streamed onto the resampling grid as the steps end) against the oracle's
+recorded-then-resampled pulse, the reference's NHTSA validations on the GPU
+path (tests/impact.rs:192-211, 284-309), and the batched and served paths
+against the one-model path."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import crushrs_amd as ca
+from crushrs_amd import signal as sg
+import oracle_ffi as of
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).resolve().parent / "golden" / "nhtsa"
+
+
+def pulse_parity(g, o, dv_tol=0.05):
+    """Same grid length (to one sample at the end, where the last step's
+    time may straddle a grid point), CFC 60 pulses within 1 % of the peak,
+    pulse delta-v within dv_tol."""
+    assert g.pulse is not None and o.pulse is not None
+    n = min(g.pulse.shape[1], o.pulse.shape[1])
+    assert abs(g.pulse.shape[1] - o.pulse.shape[1]) <= 1
+    for k in range(g.pulse.shape[0]):
+        fg = sg.cfc_filter(g.pulse[k, :n], g.pulse_dt, 60.0)
+        fo = sg.cfc_filter(o.pulse[k, :n], o.pulse_dt, 60.0)
+        peak = np.abs(fo).max()
+        assert np.abs(fg - fo).max() < 1e-2 * peak, (k, np.abs(fg - fo).max(), peak)
+    for a, b in zip(g.groups, o.groups):
+        assert abs(a.pulse_delta_v - b.pulse_delta_v) < dv_tol, (a.pulse_delta_v, b.pulse_delta_v)
+
+
+def test_crash_pulse_matches_oracle():
+    print("case: crash 30 30 pulse vs oracle", flush=True)
+    m = ca.Model.crash(30.0, 30.0)
+    pulse_parity(ca.run(m), of.run(m))
+
+
+def test_nhtsa_4429_pulse_delta_v_on_gpu():
+    """tests/impact.rs:284-309: CFC 60 rear-accelerometer delta-v of
+    Navigator vs pulse-tuned Neon (1378 kg) at 30.14 mph each: 9.89 +- 1.0
+    and 19.57 +- 2.0 m/s; the GPU within 0.05 m/s of the oracle."""
+    print("case: NHTSA 4429 head-on pulse", flush=True)
+    a = ca.renamed(ca.vehicle("navigator"), "navigator_a")
+    b = ca.vehicle("neon-pulse")
+    b.mass = 1378.0
+    b = ca.renamed(b, "neon_b")
+    v0 = 30.14 * ca.MPH
+    m = ca.Model.headon(a, b, v0, v0)
+    g, o = ca.run(m), of.run(m)
+    assert abs(g.groups[0].pulse_delta_v - 9.89) < 1.0
+    assert abs(g.groups[1].pulse_delta_v - 19.57) < 2.0
+    pulse_parity(g, o)
+
+
+def test_nhtsa_2320_barrier_pulse_on_gpu():
+    """tests/impact.rs:192-211 with the GPU's pulse: the pulse-calibrated
+    Neon at 35 mph into a rigid barrier against the measured NHTSA 2320
+    rear-seat pulse."""
+    print("case: NHTSA 2320 barrier pulse", flush=True)
+    v = ca.vehicle("neon-pulse")
+    speed = 35.0 * ca.MPH
+    m = ca.Model.barrier(v, element_size=ca.TUNED_ELEMENT_SIZE, speed=speed, end_time=sg.PULSE_END)
+    r = ca.run(m)
+    sim = sg.Pulse.from_results(r, 0).filtered(60.0)
+    meas = sg.Pulse.read_nhtsa_tsv(GOLDEN / "v02320tsv.078", True).filtered(60.0).resampled(
+        sg.PULSE_DT).truncated(sg.PULSE_END)
+    cmp = sg.PulseComparison.new(meas, sim, v.mass, speed)
+    dv_meas, dv_sim = speed - cmp.v_meas[-1], speed - cmp.v_sim[-1]
+    assert abs(dv_sim - dv_meas) < 0.05 * dv_meas
+    assert abs(cmp.sim.peak_accel / cmp.meas.peak_accel - 1.0) < 0.2
+    assert abs(cmp.sim.max_crush / cmp.meas.max_crush - 1.0) < 0.15
+    assert cmp.rms_accel_error() < 8.0 * 9.81
+    assert np.abs(cmp.v_meas - cmp.v_sim).max() < 1.5
+    pulse_parity(r, of.run(m))
+
+
+def test_batch_and_server_pulses_equal_plain():
+    """Pulses and pulse delta-v of batched and served solves are bitwise
+    those of the one-model path."""
+    print("case: batch and server pulses", flush=True)
+    models = [ca.Model.crash(30.0, 30.0), ca.Model.barrier(ca.vehicle("neon-pulse"), end_time=0.05),
+              ca.Model.crash(20.0, 35.0).set(end_time=0.04)]
+    plain = [ca.run(m) for m in models]
+    b = ca.Batch(models, steps_per_graph=16)
+    b.run()
+    srv = ca.Server(hbm_budget_bytes=2 ** 30, steps_per_graph=8)
+    tickets = [srv.submit(m) for m in models]
+    served = [srv.wait(t) for t in tickets]
+    srv.close()
+    for p, r in [(p, b.result(i)) for i, p in enumerate(plain)] + list(zip(plain, served)):
+        assert np.array_equal(p.pulse, r.pulse)
+        assert [g.pulse_delta_v for g in p.groups] == [g.pulse_delta_v for g in r.groups]
