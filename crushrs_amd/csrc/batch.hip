@@ -1117,7 +1117,7 @@ void batch_result(BatchImpl& bi, int i, crush_results* out) {
             gr.crush_end = g < 8 ? r.crush_end[g] : 0.0;
             gr.crush_max = g < 8 ? r.crush_max[g] : 0.0;
         }
-    pulse_results(P, reinterpret_cast<const double*>(base + o.pulse), r.pulse_samples, out);
+    pulse_results(P, reinterpret_cast<const double*>(base + o.pulse), r.pulse_samples, r.t, out);
 }
 
 void batch_free(BatchImpl* bi) { delete bi; }

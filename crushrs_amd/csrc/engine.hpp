@@ -20,7 +20,7 @@ constexpr int BATCH_GRID_ENTRIES = 4096;
 // their grids then move to a kernel with a 64 KB LDS grid (BIG_GRID_CELLS
 // cells, BIG_GRID_ENTRIES entries), and past that to a device-memory region
 // of 8 cells and 64 entries per face (at least the large LDS capacities).
-constexpr int BATCH_GRID_LARGE = 512;
+constexpr int BATCH_GRID_LARGE = 400;
 constexpr int BIG_GRID_CELLS = 2048;
 constexpr int BIG_GRID_ENTRIES = 12000;
 constexpr int NODE_TILE = 256;

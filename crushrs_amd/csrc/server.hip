@@ -377,7 +377,7 @@ static void fill(const Request& r, const Prepared& P, crush_results* out) {
             gr.crush_end = g < 8 ? r.ctl.crush_end[g] : 0.0;
             gr.crush_max = g < 8 ? r.ctl.crush_max[g] : 0.0;
         }
-    pulse_results_host(P, r.pulse.empty() ? nullptr : r.pulse.data(), r.ctl.pulse_k, out);
+    pulse_results_host(P, r.pulse.empty() ? nullptr : r.pulse.data(), r.ctl.pulse_k, r.ctl.t, out);
 }
 
 // Hand a finished request to the caller and release its ticket (mu held).

@@ -97,7 +97,12 @@ double pulse_velocity_change(const double* accel, size_t n, double dt, double cf
     return acc;
 }
 
-void pulse_results_host(const Prepared& P, const double* h, int samples, crush_results* out) {
+void pulse_results_host(const Prepared& P, const double* h, int samples, double t_end, crush_results* out) {
+    // the resampling grid of Pulse::from_history ends at floor(t_end / dt)
+    // (src/signal.rs:44); the device streams every k dt <= t, which can add
+    // the point at t_end when k dt rounds onto it
+    if (P.n_pulse > 0 && P.settings.pulse_dt > 0.0)
+        samples = std::min(samples, (int)std::floor((t_end - 0.0) / P.settings.pulse_dt) + 1);
     if (P.n_pulse <= 0 || samples <= 0 || !h) {
         out->pulse_samples = 0;
         if (out->groups)
@@ -132,9 +137,9 @@ std::vector<double> pulse_download(const Prepared& P, const double* dev_pulse) {
     return h;
 }
 
-void pulse_results(const Prepared& P, const double* dev_pulse, int samples, crush_results* out) {
+void pulse_results(const Prepared& P, const double* dev_pulse, int samples, double t_end, crush_results* out) {
     const std::vector<double> h = pulse_download(P, dev_pulse);
-    pulse_results_host(P, h.empty() ? nullptr : h.data(), samples, out);
+    pulse_results_host(P, h.empty() ? nullptr : h.data(), samples, t_end, out);
 }
 
 }  // namespace crush

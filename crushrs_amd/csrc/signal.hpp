@@ -28,9 +28,11 @@ struct Prepared;
 // ([min(G, 8)][P.n_pulse] doubles at dev_pulse, `samples` filled): the
 // CFC 60 pulse delta-v per group into out->groups (when given) and the raw
 // pulses into out->pulse (when given).
-void pulse_results(const Prepared& P, const double* dev_pulse, int samples, crush_results* out);
+// t_end is the time the solve reached.
+void pulse_results(const Prepared& P, const double* dev_pulse, int samples, double t_end, crush_results* out);
 // The same from a host copy of the pulses (pulse_download).
-void pulse_results_host(const Prepared& P, const double* host_pulse, int samples, crush_results* out);
+void pulse_results_host(const Prepared& P, const double* host_pulse, int samples, double t_end,
+                        crush_results* out);
 std::vector<double> pulse_download(const Prepared& P, const double* dev_pulse);
 
 }  // namespace crush

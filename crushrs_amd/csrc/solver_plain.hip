@@ -416,7 +416,7 @@ static void fill_results(const Prepared& P, PlainSolver& S, const StepCtl& c, cr
             r.crush_max = g < 8 ? std::max(c.crush_max[g], cr) : cr;
         }
     }
-    pulse_results(P, S.D->pulse.p, c.pulse_k, out);
+    pulse_results(P, S.D->pulse.p, c.pulse_k, c.t, out);
 }
 
 // Guard against a solve that never reaches end_time (dt is floored at 1e-15).
