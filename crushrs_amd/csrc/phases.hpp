@@ -214,14 +214,35 @@ __device__ __forceinline__ int contact_node(const InstDev& D, const ContactDesc&
             ci[d] = min(max((int)floor(r), 0), g.dim[d] - 1);
         }
         if (inside) {
+            // candidates four at a time: their ids and candidate boxes are
+            // loaded together, and a node outside a face's candidate box
+            // (a superset of what the face can accept) skips the test
             const int cell = (ci[2] * g.dim[1] + ci[1]) * g.dim[0] + ci[0];
-            for (int q = s_start[cell]; q < s_start[cell + 1]; ++q) {
-                const int f = s_entries[q];
-                if (best >= 0 && f >= best) continue;
-                double sd;
-                if (face_accepts(D.fgeo, F, fb + f, p, cd.max_distance, &sd)) {
-                    best = f;
-                    best_sd = sd;
+            const int qe = s_start[cell + 1];
+            for (int q0 = s_start[cell]; q0 < qe; q0 += 4) {
+                int fs[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) fs[j] = q0 + j < qe ? s_entries[q0 + j] : -1;
+                bool in_box[4];
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int gf = fb + (fs[j] >= 0 ? fs[j] : 0);
+                    bool ok = fs[j] >= 0;
+#pragma unroll
+                    for (int d = 0; d < 3; ++d)
+                        ok = ok && p[d] >= D.fgeo[(size_t)(FG_CLO + d) * F + gf] &&
+                             p[d] <= D.fgeo[(size_t)(FG_CHI + d) * F + gf];
+                    in_box[j] = ok;
+                }
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int f = fs[j];
+                    if (!in_box[j] || (best >= 0 && f >= best)) continue;
+                    double sd;
+                    if (face_accepts(D.fgeo, F, fb + f, p, cd.max_distance, &sd)) {
+                        best = f;
+                        best_sd = sd;
+                    }
                 }
             }
         }
@@ -593,11 +614,12 @@ __device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, dou
     }
     sz = block_reduce<B>(sz, 2, s_w);
     double h = fmax(sz * cell_scale, 1e-9);
-    // cell size: grown until the grid has at most MAXC cells, then doubled
-    // (at most 4 times) while the faces' candidate boxes would make more
-    // than MAXE entries
-    long long cells = 1;
+    // cell size: grown until the grid has at most MAXC cells; when the
+    // faces' candidate boxes then make more than MAXE entries, doubled and
+    // counted again (at most 4 times)
+    int ncell = 1;
     for (int attempt = 0;; ++attempt) {
+        long long cells = 1;
         for (int iter = 0; iter < 64; ++iter) {
             cells = 1;
 #pragma unroll
@@ -610,51 +632,38 @@ __device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, dou
             h *= 1.25;
         }
         g.h = h;
-        if (attempt == 4 || cells > MAXC || nf == 0 || !(isfinite(g.glo[0]) && isfinite(ghi[0]))) break;
-        double entries = 0.0;
-        for (int f = t; f < nf; f += B) {
-            double flo[3], fhi[3];
+        g.mode = 0;
+        g.reserved = 0;
+        g.overflow = (nf == 0 || cells > MAXC || !(isfinite(g.glo[0]) && isfinite(ghi[0]))) ? 1 : 0;
+        const int dx = g.dim[0], dy = g.dim[1];
+        ncell = g.overflow ? 1 : (int)cells;
+        __syncthreads();
+        for (int i = t; i < ncell + 1; i += B) s_start[i] = 0;
+        __syncthreads();
+        if (!g.overflow) {
+            for (int f = t; f < nf; f += B) {
+                double flo[3], fhi[3];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                flo[d] = D.fgeo[(size_t)(FG_CLO + d) * F + fb + f];
-                fhi[d] = D.fgeo[(size_t)(FG_CHI + d) * F + fb + f];
+                for (int d = 0; d < 3; ++d) {
+                    flo[d] = D.fgeo[(size_t)(FG_CLO + d) * F + fb + f];
+                    fhi[d] = D.fgeo[(size_t)(FG_CHI + d) * F + fb + f];
+                }
+                if (!(flo[0] <= fhi[0] && flo[1] <= fhi[1] && flo[2] <= fhi[2])) continue;
+                const int a0 = grid_cell(g, flo[0], 0), a1 = grid_cell(g, fhi[0], 0);
+                const int b0 = grid_cell(g, flo[1], 1), b1 = grid_cell(g, fhi[1], 1);
+                const int c0 = grid_cell(g, flo[2], 2), c1 = grid_cell(g, fhi[2], 2);
+                for (int kz = c0; kz <= c1; ++kz)
+                    for (int ky = b0; ky <= b1; ++ky)
+                        for (int kx = a0; kx <= a1; ++kx) atomicAdd(&s_start[(kz * dy + ky) * dx + kx], 1);
             }
-            if (!(flo[0] <= fhi[0] && flo[1] <= fhi[1] && flo[2] <= fhi[2])) continue;
-            entries += (double)(grid_cell(g, fhi[0], 0) - grid_cell(g, flo[0], 0) + 1) *
-                       (double)(grid_cell(g, fhi[1], 1) - grid_cell(g, flo[1], 1) + 1) *
-                       (double)(grid_cell(g, fhi[2], 2) - grid_cell(g, flo[2], 2) + 1);
         }
-        entries = block_reduce<B>(entries, 0, s_w);
-        if (entries <= (double)MAXE) break;
+        __syncthreads();
+        block_scan<B>(s_start, ncell + 1, s_tmp);
+        if (g.overflow || s_start[ncell] <= MAXE || attempt == 4) break;
         h *= 2.0;
     }
-    g.mode = 0;
-    g.reserved = 0;
-    g.overflow = (nf == 0 || cells > MAXC || !(isfinite(g.glo[0]) && isfinite(ghi[0]))) ? 1 : 0;
-    const int dx = g.dim[0], dy = g.dim[1], dz = g.dim[2];
-    const int ncell = g.overflow ? 1 : dx * dy * dz;
-    for (int i = t; i < ncell + 1; i += B) s_start[i] = 0;
-    __syncthreads();
-    if (!g.overflow) {
-        for (int f = t; f < nf; f += B) {
-            double flo[3], fhi[3];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                flo[d] = D.fgeo[(size_t)(FG_CLO + d) * F + fb + f];
-                fhi[d] = D.fgeo[(size_t)(FG_CHI + d) * F + fb + f];
-            }
-            if (!(flo[0] <= fhi[0] && flo[1] <= fhi[1] && flo[2] <= fhi[2])) continue;
-            const int a0 = grid_cell(g, flo[0], 0), a1 = grid_cell(g, fhi[0], 0);
-            const int b0 = grid_cell(g, flo[1], 1), b1 = grid_cell(g, fhi[1], 1);
-            const int c0 = grid_cell(g, flo[2], 2), c1 = grid_cell(g, fhi[2], 2);
-            for (int kz = c0; kz <= c1; ++kz)
-                for (int ky = b0; ky <= b1; ++ky)
-                    for (int kx = a0; kx <= a1; ++kx) atomicAdd(&s_start[(kz * dy + ky) * dx + kx], 1);
-        }
-    }
-    __syncthreads();
-    block_scan<B>(s_start, ncell + 1, s_tmp);
     if (s_start[ncell] > MAXE) g.overflow = 1;
+    const int dx = g.dim[0], dy = g.dim[1];
     for (int i = t; i < ncell; i += B) s_cursor[i] = s_start[i];
     __syncthreads();
     if (!g.overflow) {
