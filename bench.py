@@ -1,7 +1,7 @@
 """Benchmark of the crash-solver core on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W]
-                    [--workload sweep|calib|large|crash|serve]
+                    [--workload sweep|calib|large|crash|serve|pulse-fit]
                     [--instances N] [--no-cpu] [--no-plain]
 
 Workloads
@@ -24,6 +24,10 @@ Workloads
   serve  Poisson arrivals of delta-v requests (head-on/offset, T-bone,
          rear-end) into the continuous-batching server, at several offered
          loads; value is the highest achieved solves/s.
+  pulse-fit  the pulse calibration driver (calibrate_pulse) on NHTSA test
+         2320: one step is one pattern-search round, one batch of 18
+         barrier probes, timed end to end (model build, upload, solve,
+         pulse scoring).
 
 Prints one JSON line.  `value` is solves/s of the workload with every model
 already resident in device memory; preparation and upload are reported
@@ -549,12 +553,94 @@ def bench_crash(args):
     return out
 
 
+def bench_pulse_fit(args):
+    """The pulse calibration driver as a batched-solver client: rounds of
+    calibrate_pulse's pattern search on NHTSA test 2320 (the pulse-tuned
+    Neon from its measured-curve start, 35 mph rigid barrier, 0.15 s), each
+    round one batch of 18 barrier probes.  Timed end to end per round: the
+    host builds the probe models, the batch uploads and solves them, the
+    pulses come back and are scored."""
+    from crushrs_amd import calibration as cal
+    from crushrs_amd import signal as sg
+    meas = sg.Pulse.read_nhtsa_tsv(ROOT / "tests" / "golden" / "nhtsa" / "v02320tsv.078", True)
+    speed = 35.0 * ca.MPH
+    v = ca.vehicle("neon-pulse")
+    v.n_force_table = 0
+    timing = {"build": 0.0, "upload": 0.0, "solve": 0.0, "results": 0.0}
+    solves = [0]
+
+    def timed_solve(models):
+        t0 = time.perf_counter()
+        b = ca.Batch(models, steps_per_graph=args.steps_per_graph)
+        t1 = time.perf_counter()
+        b.run()
+        t2 = time.perf_counter()
+        out = [b.result(i, fields=False) for i in range(len(models))]
+        t3 = time.perf_counter()
+        timing["upload"] += t1 - t0
+        timing["solve"] += t2 - t1
+        timing["results"] += t3 - t2
+        solves[0] += len(models)
+        return out
+
+    # warm-up: the first round (start + probes) loads kernels
+    cal.calibrate_pulse(v, meas, speed, rounds=max(args.warmup, 1))
+    for k in timing:
+        timing[k] = 0.0
+    solves[0] = 0
+    t0 = time.perf_counter()
+    fit = cal.calibrate_pulse(v, meas, speed, rounds=args.steps, solve=timed_solve)
+    wall = time.perf_counter() - t0
+    value = solves[0] / wall
+    probe_models = [ca.Model.barrier(cal.pulse_candidate(fit.vehicle, p, 1.1) or fit.vehicle, speed=speed)
+                    for p in range(cal.N_KNOTS + 2)]
+    steps = ca.run(probe_models[0]).steps
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / fit.rounds,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "measured: NHTSA test 2320 rear-seat X pulse (the reference's test record); generated barrier models",
+        "config": {"workload": ("pulse calibration driver: calibrate_pulse pattern search on NHTSA 2320, pulse-tuned "
+                                "Neon from its measured-curve start, 35 mph rigid barrier, 0.15 s; one step = one "
+                                "round = one batch of 18 probes (the first round also solves the start)"),
+                   "rounds": fit.rounds, "solves": solves[0], "elements": probe_models[0].n_hexes,
+                   "solver_steps": int(steps)},
+        "round_ms": {k: 1e3 * t / fit.rounds for k, t in timing.items()},
+        "objective": {"start": fit.start_objective, "end": fit.objective},
+        "roofline": None,
+    }
+    out["round_ms"]["host_search"] = 1e3 * wall / fit.rounds - sum(out["round_ms"].values())
+    if not args.no_plain:
+        s0 = time.perf_counter()
+        for m in probe_models:
+            ca.run(m)
+        tp = (time.perf_counter() - s0) / len(probe_models)
+        out["plain_gpu"] = {"value": 1.0 / tp, "unit": "solves/s",
+                            "sample": f"{len(probe_models)} probe models solved one at a time"}
+        out["speedup_vs_plain"] = value * tp
+    if not args.no_cpu:
+        es = float(probe_models[0].n_hexes * steps)
+        out["cpu_baseline"] = cpu_baseline(probe_models, es, budget_s=15.0)
+        out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
+    else:
+        out["cpu_baseline"] = None
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="sweep", choices=["sweep", "calib", "large", "crash", "serve"])
+    ap.add_argument("--workload", default="sweep", choices=["sweep", "calib", "large", "crash", "serve", "pulse-fit"])
     ap.add_argument("--rates", default="10,50,200,800,2000", help="serve: offered loads, requests/s")
     ap.add_argument("--requests", type=int, default=400, help="serve: requests per offered load")
     ap.add_argument("--top-requests", type=int, default=2000, help="serve: requests at the highest offered load")
@@ -603,6 +689,10 @@ def main():
         args.steps = 3 if args.steps is None else args.steps
         args.warmup = 1 if args.warmup is None else args.warmup
         out = bench_large(args)
+    elif args.workload == "pulse-fit":
+        args.steps = 4 if args.steps is None else args.steps
+        args.warmup = 1 if args.warmup is None else args.warmup
+        out = bench_pulse_fit(args)
     else:
         args.steps = 20 if args.steps is None else args.steps
         args.warmup = 2 if args.warmup is None else args.warmup
