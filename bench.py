@@ -131,7 +131,7 @@ def cpu_baseline(models, elem_steps_per_solve: float, budget_s: float = 20.0, sa
             busy += max((t2 - t1) - (t1 - t0), 1e-9)
             es += m.n_hexes * done.value
             k += 1
-        return es / busy, k
+        return (es / busy if busy > 0.0 else 0.0), k
 
     with ThreadPoolExecutor(max_workers=cores) as ex:
         res = list(ex.map(worker, range(cores)))

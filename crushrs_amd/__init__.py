@@ -185,6 +185,14 @@ class Model:
         check(self._lib.crush_contact_node_count(self.ptr, C.byref(n)))
         return n.value
 
+    def output_capacity(self) -> tuple[int, int]:
+        """(history samples, frames) a solve of this model can report."""
+        if self.settings.history_steps <= 0 and self.settings.frame_steps <= 0:
+            return 0, 0
+        h, f = C.c_int32(), C.c_int32()
+        check(self._lib.crush_output_capacity(self.ptr, C.byref(h), C.byref(f)))
+        return h.value, f.value
+
 
 @dataclass
 class GroupResult:
@@ -213,6 +221,22 @@ class Results:
     # t = k * pulse_dt (None when the model records none)
     pulse: Optional[np.ndarray] = None
     pulse_dt: float = 0.0
+    # histories (settings.history_steps > 0): sample times and steps [K],
+    # part states [K][n_parts][PART_FIELDS] (mass, u xyz, v xyz, a xyz,
+    # kinetic energy), accelerometer samples [K][min(n_groups, 8)][ACCEL_FIELDS]
+    # (position, u, v, a xyz each, frame origin xyz, frame axes 3x3 rows)
+    history_time: Optional[np.ndarray] = None
+    history_step: Optional[np.ndarray] = None
+    parts: Optional[np.ndarray] = None
+    accel: Optional[np.ndarray] = None
+    # frames (settings.frame_steps > 0, one-model path): times [F],
+    # displacement [F][n_nodes][3], plastic strain [F][n_hexes], Cauchy
+    # stress [F][n_hexes][6] (xx yy zz xy yz xz), eroded [F][n_hexes]
+    frame_time: Optional[np.ndarray] = None
+    frame_displacement: Optional[np.ndarray] = None
+    frame_plastic_strain: Optional[np.ndarray] = None
+    frame_stress: Optional[np.ndarray] = None
+    frame_eroded: Optional[np.ndarray] = None
 
     def mean_velocity(self, nodes) -> tuple[float, np.ndarray]:
         """Results::mean_velocity / Model::mean_velocity (src/model.rs:281-295)."""
@@ -227,7 +251,8 @@ class Results:
 PULSE_CAPACITY = 8192  # samples per group kept in Results.pulse (0.8 s at 1e-4 s)
 
 
-def alloc_results(n_nodes: int, n_groups: int, pulse_capacity: int = PULSE_CAPACITY):
+def alloc_results(n_nodes: int, n_groups: int, pulse_capacity: int = PULSE_CAPACITY, n_parts: int = 0,
+                  history_capacity: int = 0, n_hexes: int = 0, frame_capacity: int = 0):
     u = np.zeros((n_nodes, 3))
     v = np.zeros((n_nodes, 3))
     m = np.zeros(n_nodes)
@@ -240,26 +265,69 @@ def alloc_results(n_nodes: int, n_groups: int, pulse_capacity: int = PULSE_CAPAC
     p = np.zeros((max(min(n_groups, 8), 1), pulse_capacity))
     r.pulse = p.ctypes.data_as(_abi.f64p)
     r.pulse_capacity = pulse_capacity
-    r.pulse_buffer = p  # keeps the array alive with the struct
+    bufs = {"pulse": p}  # kept alive with the struct
+    if history_capacity > 0:
+        k = history_capacity
+        bufs["history_time"] = np.zeros(k)
+        bufs["history_step"] = np.zeros(k, np.int64)
+        bufs["parts"] = np.zeros((k, max(n_parts, 1), _abi.PART_FIELDS))
+        bufs["accel"] = np.zeros((k, max(min(n_groups, 8), 1), _abi.ACCEL_FIELDS))
+        r.history_time = bufs["history_time"].ctypes.data_as(_abi.f64p)
+        r.history_step = bufs["history_step"].ctypes.data_as(C.POINTER(C.c_int64))
+        r.history = bufs["parts"].ctypes.data_as(_abi.f64p)
+        r.accel_history = bufs["accel"].ctypes.data_as(_abi.f64p)
+        r.history_capacity = k
+    if frame_capacity > 0:
+        f = frame_capacity
+        bufs["frame_time"] = np.zeros(f)
+        bufs["frame_displacement"] = np.zeros((f, n_nodes, 3), np.float32)
+        bufs["frame_plastic_strain"] = np.zeros((f, n_hexes), np.float32)
+        bufs["frame_stress"] = np.zeros((f, n_hexes, 6), np.float32)
+        bufs["frame_eroded"] = np.zeros((f, n_hexes), np.uint8)
+        r.frame_time = bufs["frame_time"].ctypes.data_as(_abi.f64p)
+        r.frame_displacement = bufs["frame_displacement"].ctypes.data_as(C.POINTER(C.c_float))
+        r.frame_plastic_strain = bufs["frame_plastic_strain"].ctypes.data_as(C.POINTER(C.c_float))
+        r.frame_stress = bufs["frame_stress"].ctypes.data_as(C.POINTER(C.c_float))
+        r.frame_eroded = bufs["frame_eroded"].ctypes.data_as(_abi.u8p)
+        r.frame_capacity = f
+    r.buffers = bufs
     return r, u, v, m, g
+
+
+def alloc_model_results(model: "Model"):
+    """alloc_results sized for everything a solve of `model` reports."""
+    hc, fc = model.output_capacity()
+    v = model.view
+    return alloc_results(v.n_nodes, v.n_groups, PULSE_CAPACITY, v.n_parts, hc, v.n_hexes, fc)
 
 
 def to_results(r, u, v, m, g, pulse_dt: float = 0.0) -> Results:
     groups = [GroupResult(g[i].mass, np.array(g[i].delta_v[:]), g[i].crush_end, g[i].crush_max, g[i].pulse_delta_v)
               for i in range(r.n_groups)]
+    bufs = getattr(r, "buffers", {})
     pulse = None
-    buf = getattr(r, "pulse_buffer", None)
-    if buf is not None and r.pulse_samples > 0:
+    if "pulse" in bufs and r.pulse_samples > 0:
         ns = min(int(r.pulse_samples), r.pulse_capacity)
-        pulse = buf[:min(r.n_groups, 8), :ns].copy()
-    return Results(r.time, int(r.steps), int(r.eroded_elements), r.wall_ms, r.device_ms, r.added_mass_max,
-                   bool(r.separated), u, v, m, groups, pulse, pulse_dt)
+        pulse = bufs["pulse"][:min(r.n_groups, 8), :ns].copy()
+    out = Results(r.time, int(r.steps), int(r.eroded_elements), r.wall_ms, r.device_ms, r.added_mass_max,
+                  bool(r.separated), u, v, m, groups, pulse, pulse_dt)
+    if "history_time" in bufs and r.history_samples > 0:
+        k = min(int(r.history_samples), r.history_capacity)
+        out.history_time = bufs["history_time"][:k].copy()
+        out.history_step = bufs["history_step"][:k].copy()
+        out.parts = bufs["parts"][:k].copy()
+        out.accel = bufs["accel"][:k, :min(r.n_groups, 8)].copy()
+    if "frame_time" in bufs and r.frame_samples > 0:
+        f = min(int(r.frame_samples), r.frame_capacity)
+        for name in ("frame_time", "frame_displacement", "frame_plastic_strain", "frame_stress", "frame_eroded"):
+            setattr(out, name, bufs[name][:f].copy())
+    return out
 
 
 def run(model: Model) -> Results:
     """crushrs::run on the GPU (src/solver.rs:190-381)."""
     lib = load()
-    r, u, v, m, g = alloc_results(model.n_nodes, model.view.n_groups)
+    r, u, v, m, g = alloc_model_results(model)
     check(lib.crush_run(model.ptr, C.byref(r)))
     return to_results(r, u, v, m, g, model.settings.pulse_dt)
 
@@ -334,7 +402,9 @@ class Batch:
         return o
 
     def result(self, i: int, fields: bool = True) -> Results:
-        r, u, v, m, g = alloc_results(self._n_nodes[i], self._n_groups[i])
+        mdl = self.models[i]
+        hc = mdl.output_capacity()[0] if mdl.settings.history_steps > 0 else 0
+        r, u, v, m, g = alloc_results(self._n_nodes[i], self._n_groups[i], PULSE_CAPACITY, mdl.view.n_parts, hc)
         if not fields:
             r.displacement = None
             r.velocity = None
@@ -380,13 +450,15 @@ class Server:
     def submit(self, model: Model, fields: bool = False) -> int:
         t = C.c_int64()
         check(self._lib.crush_server_submit(self._h, model.ptr, _abi.SUBMIT_FIELDS if fields else 0, C.byref(t)))
-        self._shape[t.value] = (model.n_nodes, model.view.n_groups, fields, model.settings.pulse_dt)
+        hc = model.output_capacity()[0] if model.settings.history_steps > 0 else 0
+        self._shape[t.value] = (model.n_nodes, model.view.n_groups, fields, model.settings.pulse_dt,
+                                model.view.n_parts, hc)
         return t.value
 
     def _alloc(self, ticket):
         # an unknown ticket gets empty outputs; the library reports the error
-        n_nodes, n_groups, fields, pulse_dt = self._shape.get(ticket, (0, 0, False, 0.0))
-        r, u, v, m, g = alloc_results(n_nodes, n_groups)
+        n_nodes, n_groups, fields, pulse_dt, n_parts, hc = self._shape.get(ticket, (0, 0, False, 0.0, 0, 0))
+        r, u, v, m, g = alloc_results(n_nodes, n_groups, PULSE_CAPACITY, n_parts, hc)
         if not fields:
             r.displacement = None
             r.velocity = None

@@ -66,6 +66,8 @@ class Settings(C.Structure):
         ("adaptive_check_steps", C.c_int32),
         ("separation_steps", C.c_int32),
         ("pulse_dt", C.c_double),
+        ("history_steps", C.c_int32),
+        ("frame_steps", C.c_int32),
     ]
 
 
@@ -80,7 +82,7 @@ class Group(C.Structure):
         ("frame_origin", C.c_int32),
         ("frame_x", C.c_int32),
         ("heading", C.c_int32),
-        ("reserved0", C.c_int32),
+        ("frame_plane", C.c_int32),
     ]
 
 
@@ -134,7 +136,24 @@ class Results(C.Structure):
         ("pulse", f64p),
         ("pulse_capacity", C.c_int32),
         ("pulse_samples", C.c_int32),
+        ("history_time", f64p),
+        ("history_step", C.POINTER(C.c_int64)),
+        ("history", f64p),
+        ("accel_history", f64p),
+        ("history_capacity", C.c_int32),
+        ("history_samples", C.c_int32),
+        ("frame_time", f64p),
+        ("frame_displacement", C.POINTER(C.c_float)),
+        ("frame_plastic_strain", C.POINTER(C.c_float)),
+        ("frame_stress", C.POINTER(C.c_float)),
+        ("frame_eroded", u8p),
+        ("frame_capacity", C.c_int32),
+        ("frame_samples", C.c_int32),
     ]
+
+
+PART_FIELDS = 11   # CRUSH_PART_FIELDS
+ACCEL_FIELDS = 24  # CRUSH_ACCEL_FIELDS
 
 
 class Vehicle(C.Structure):
@@ -274,6 +293,7 @@ SIGNATURES = {
     "crush_run": (C.c_int32, [C.POINTER(ModelView), C.POINTER(Results)]),
     "crush_debug_run_steps": (C.c_int32, [C.POINTER(ModelView), C.c_int32, f64p, f64p, f64p, i32p, u8p, f64p, i32p]),
     "crush_contact_node_count": (C.c_int32, [C.POINTER(ModelView), i32p]),
+    "crush_output_capacity": (C.c_int32, [C.POINTER(ModelView), i32p, i32p]),
     "crush_batch_defaults": (None, [C.POINTER(BatchOptions)]),
     "crush_batch_create": (C.c_int32, [C.POINTER(C.POINTER(ModelView)), C.c_int32, C.POINTER(BatchOptions),
                                        C.POINTER(C.c_void_p)]),

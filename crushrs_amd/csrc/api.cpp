@@ -54,4 +54,15 @@ int32_t crush_contact_node_count(const crush_model* model, int32_t* count) {
     });
 }
 
+int32_t crush_output_capacity(const crush_model* model, int32_t* history_samples, int32_t* frame_samples) {
+    return guarded([&] {
+        if (!model) fail(CRUSH_E_INPUT, "null argument");
+        Prepared p;
+        prepare(*model, p);
+        if (history_samples) *history_samples = p.settings.history_steps > 0 ? 1 + p.n_hist : 0;
+        if (frame_samples)
+            *frame_samples = p.settings.frame_steps > 0 ? 1 + history_capacity(p, p.settings.frame_steps) : 0;
+    });
+}
+
 }  // extern "C"

@@ -22,6 +22,7 @@
 // captured in one hipGraph and replayed until every instance has retired;
 // retired instances leave the active lists between replays.
 #include "engine.hpp"
+#include "outputs.hpp"
 #include "signal.hpp"
 
 #include <hip/hip_runtime.h>
@@ -159,6 +160,8 @@ ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     o.g_v0 = b.put(P.g_v0);
     o.g_dv = b.zeros(sizeof(double) * 3 * std::max(P.n_groups, 1));
     o.g_frame = b.put(P.g_frame);
+    o.p_nodes_ptr = b.put(P.p_nodes_ptr);
+    o.p_nodes = b.put(P.p_nodes);
     o.cgp = b.zeros(sizeof(FaceGrid) * std::max<size_t>(P.cdesc.size(), 1));
     {
         StepCtl c = initial_ctl(P);
@@ -184,6 +187,8 @@ ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     // pulses: filled in order from sample 0 as the steps end, never read
     // before they are written
     o.pulse = scratch(sizeof(double) * (size_t)std::min(P.n_groups, 8) * (size_t)P.n_pulse);
+    // history records, written in order as the samples are taken
+    o.hist = scratch(sizeof(double) * hist_record_len(P.n_parts, P.n_groups) * (size_t)P.n_hist);
     o.total = off;
     out.swap(b.bytes);
     return o;
@@ -255,6 +260,12 @@ InstDev make_inst(const Prepared& P, const ImageOffsets& o, uintptr_t base) {
     I.pulse = at<double>(base, o.pulse);
     I.n_pulse = P.n_pulse;
     I.pulse_dt = P.settings.pulse_dt;
+    I.p_nodes_ptr = at<int32_t>(base, o.p_nodes_ptr);
+    I.p_nodes = at<int32_t>(base, o.p_nodes);
+    I.hist = at<double>(base, o.hist);
+    I.n_parts = P.n_parts;
+    I.n_hist = P.n_hist;
+    I.hist_steps = std::max(P.settings.history_steps, 1);
     I.ctl = at<StepCtl>(base, o.ctl);
     I.cgrid = at<int32_t>(base, o.cgrid);
     I.cgp = at<FaceGrid>(base, o.cgp);
@@ -597,6 +608,7 @@ __global__ void k_collect(const InstDev* __restrict__ table, int n, BatchSummary
     r.error = c.error;
     r.separated = c.separated;
     r.pulse_samples = c.pulse_k;
+    r.hist_samples = c.hist_k;
     r.t_retire = c.t_retire;
     for (int g = 0; g < 8; ++g) {
         r.crush_max[g] = g < D.n_groups ? c.crush_max[g] : 0.0;
@@ -1118,6 +1130,10 @@ void batch_result(BatchImpl& bi, int i, crush_results* out) {
             gr.crush_max = g < 8 ? r.crush_max[g] : 0.0;
         }
     pulse_results(P, reinterpret_cast<const double*>(base + o.pulse), r.pulse_samples, r.t, out);
+    const std::vector<double> h =
+        history_download(P, reinterpret_cast<const double*>(base + o.hist), r.hist_samples, bi.stream);
+    history_results(P, h.data(), r.hist_samples, out);
+    out->frame_samples = 0;
 }
 
 void batch_free(BatchImpl* bi) { delete bi; }

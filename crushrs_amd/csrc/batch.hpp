@@ -15,6 +15,7 @@ struct BatchSummary {
     long long eroded;
     double added_max;
     int done, error, separated, pulse_samples;
+    int hist_samples, reserved0;
     double dv[8][3];
     double crush_end[8];
     double crush_max[8];

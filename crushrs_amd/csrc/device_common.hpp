@@ -48,7 +48,7 @@ struct StepCtl {
     double pulse_t;
     double pulse_a[8];
     int pulse_k;
-    int reserved4;
+    int hist_k;  // history records taken after the initial one (may exceed the capacity)
 };
 
 // Face geometry record width (doubles) in the SoA face array:

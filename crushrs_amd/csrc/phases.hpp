@@ -91,11 +91,17 @@ struct InstDev {
     const double* g_mass;
     const double* g_v0;   // [3G] initial mean velocities
     double* g_dv;         // [3G] delta-v written when the solve ends
-    const int32_t* g_frame;  // [2G] accelerometer frame nodes (origin, x), -1 = global x
+    const int32_t* g_frame;  // [3G] accelerometer frame nodes (origin, x, plane), -1 = none
     // accelerometer pulses [min(G, 8)][n_pulse] at t = k pulse_dt (n_pulse 0: none)
     double* pulse;
     int n_pulse;
     double pulse_dt;
+    // histories: nodes of each part (CSR) and n_hist records of
+    // hist_record_len doubles, one per sample after the initial state
+    const int32_t* p_nodes_ptr;
+    const int32_t* p_nodes;
+    double* hist;
+    int n_parts, n_hist, hist_steps;
     StepCtl* ctl;
     // batched path: per-tile partials of the stable step (min dt_e, sum of
     // scaling masses), one slot per force tile of the instance
@@ -730,7 +736,7 @@ __device__ __forceinline__ double accel_local_x(const InstDev& D, int g) {
     double a[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) a[d] = (D.f_ext[3 * (size_t)n + d] - D.f_int[3 * (size_t)n + d]) * im;
-    const int o = D.g_frame[2 * g], x = D.g_frame[2 * g + 1];
+    const int o = D.g_frame[3 * g], x = D.g_frame[3 * g + 1];
     if (o < 0) return a[0];
     double e[3];
 #pragma unroll
@@ -763,6 +769,193 @@ __device__ __forceinline__ void pulse_sample(const InstDev& D, StepCtl* c, doubl
     for (int g = 0; g < ng; ++g) c->pulse_a[g] = a[g];
 }
 
+// Output state of element slot s for a frame (src/kernel.rs:269-320):
+// the Cauchy stress in Voigt order xx yy zz xy yz xz (0 for an eroded or
+// inverted element) and the compaction (honeycomb; 0 for elastic).  The
+// honeycomb stress is R sigma R^T of its corotational state; the elastic
+// stress is the Hencky tau / J at the current displacement (f32, as the
+// force pass computes it).
+__device__ __forceinline__ void element_output(const InstDev& D, int s, float* sig, float* plastic) {
+    const int nh = D.n_hc;
+    *plastic = s < nh ? D.state[(size_t)24 * nh + s] : 0.0f;
+    for (int k = 0; k < 6; ++k) sig[k] = 0.0f;
+    if (D.eroded[s]) return;
+    float c[9];
+    if (s < nh) {
+        float R[9], sh[9];
+        for (int k = 0; k < 9; ++k) R[k] = D.state[(size_t)(9 + k) * nh + s];
+        const float* st = D.state;
+        const float s0 = st[(size_t)18 * nh + s], s1 = st[(size_t)19 * nh + s], s2 = st[(size_t)20 * nh + s];
+        const float s3 = st[(size_t)21 * nh + s], s4 = st[(size_t)22 * nh + s], s5 = st[(size_t)23 * nh + s];
+        sh[0] = s0; sh[1] = s3; sh[2] = s5;
+        sh[3] = s3; sh[4] = s1; sh[5] = s4;
+        sh[6] = s5; sh[7] = s4; sh[8] = s2;
+        float t[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) t[3 * i + j] = R[3 * i] * sh[j] + R[3 * i + 1] * sh[3 + j] + R[3 * i + 2] * sh[6 + j];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) c[3 * i + j] = t[3 * i] * R[3 * j] + t[3 * i + 1] * R[3 * j + 1] + t[3 * i + 2] * R[3 * j + 2];
+    } else {
+        const int cl = D.ecls[s];
+        const int C = D.C;
+        const int n0 = D.conn[s];
+        float hd[9];
+        for (int k = 0; k < 9; ++k) hd[k] = 0.0f;
+        for (int i = 1; i < 8; ++i) {
+            const int n = D.conn[(size_t)i * D.E + s];
+            for (int a = 0; a < 3; ++a) {
+                const float du = (float)(D.u[3 * (size_t)n + a] - D.u[3 * (size_t)n0 + a]);
+                for (int b = 0; b < 3; ++b) hd[3 * a + b] += du * D.cgrad[(size_t)(3 * i + b) * C + cl];
+            }
+        }
+        float F[9];
+        for (int k = 0; k < 9; ++k) F[k] = hd[k] + ((k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f);
+        const float J = det3f(F);
+        if (!(J > 0.0f)) return;
+        float tau[9];
+        hencky_elastic(D.mats[D.cls_mat[cl]], hd, tau);
+        for (int k = 0; k < 9; ++k) c[k] = tau[k] / J;
+    }
+    sig[0] = c[0];
+    sig[1] = c[4];
+    sig[2] = c[8];
+    sig[3] = c[1];
+    sig[4] = c[5];
+    sig[5] = c[2];
+}
+
+// Doubles per history record: time, step, CRUSH_PART_FIELDS per part and
+// CRUSH_ACCEL_FIELDS per accelerometer (groups < 8).
+__host__ __device__ __forceinline__ size_t hist_record_len(int n_parts, int n_groups) {
+    return 2 + (size_t)CRUSH_PART_FIELDS * n_parts + (size_t)CRUSH_ACCEL_FIELDS * (n_groups < 8 ? n_groups : 8);
+}
+
+// Current position of node n.
+__device__ __forceinline__ void node_position(const InstDev& D, int n, double* x) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int d = 0; d < 3; ++d) x[d] = D.X[3 * (size_t)n + d] + D.u[3 * (size_t)n + d];
+}
+
+// Acceleration (f_ext - f_int) / m of node n after a step, 0 at fixed nodes
+// (src/solver.rs:333-342).
+__device__ __forceinline__ void node_accel(const InstDev& D, int n, double* a) {
+#pragma clang fp contract(off)
+    const bool fx = D.fixed[n] != 0;
+    const double im = D.inv_mass[n];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) a[d] = fx ? 0.0 : (D.f_ext[3 * (size_t)n + d] - D.f_int[3 * (size_t)n + d]) * im;
+}
+
+// Unit axes (rows e1, e2, e3) of a body-fixed frame in the current
+// configuration (src/model.rs:30-58): e1 from the origin to the x node, e3
+// normal to the plane of the three nodes (e1 x p, with p towards the plane
+// node, or towards global X, else Y, when p is parallel to e1), e2 = e3 x e1.
+__device__ __forceinline__ void frame_axes(const InstDev& D, int o, int x, int pl, double* ax) {
+#pragma clang fp contract(off)
+    double xo[3], xx[3], xp[3];
+    node_position(D, o, xo);
+    node_position(D, x, xx);
+    double e1[3], p[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < 3; ++d) e1[d] = xx[d] - xo[d];
+    if (pl >= 0) {
+        node_position(D, pl, xp);
+        for (int d = 0; d < 3; ++d) p[d] = xp[d] - xo[d];
+    }
+    double n1 = sqrt(e1[0] * e1[0] + e1[1] * e1[1] + e1[2] * e1[2]);
+    if (n1 < 1e-300) {
+        e1[0] = 1.0;
+        e1[1] = e1[2] = 0.0;
+        n1 = 1.0;
+    }
+    for (int d = 0; d < 3; ++d) e1[d] /= n1;
+    const double pn = sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    double c[3] = {p[1] * e1[2] - p[2] * e1[1], p[2] * e1[0] - p[0] * e1[2], p[0] * e1[1] - p[1] * e1[0]};
+    if (sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]) < 1e-12 * fmax(pn, 1e-300)) {
+        const bool xa = fabs(e1[0]) < 0.9;
+        p[0] = xa ? 1.0 : 0.0;
+        p[1] = xa ? 0.0 : 1.0;
+        p[2] = 0.0;
+    }
+    double e3[3] = {e1[1] * p[2] - e1[2] * p[1], e1[2] * p[0] - e1[0] * p[2], e1[0] * p[1] - e1[1] * p[0]};
+    const double n3 = sqrt(e3[0] * e3[0] + e3[1] * e3[1] + e3[2] * e3[2]);
+    for (int d = 0; d < 3; ++d) e3[d] /= n3;
+    const double e2[3] = {e3[1] * e1[2] - e3[2] * e1[1], e3[2] * e1[0] - e3[0] * e1[2], e3[0] * e1[1] - e3[1] * e1[0]};
+    for (int d = 0; d < 3; ++d) {
+        ax[d] = e1[d];
+        ax[3 + d] = e2[d];
+        ax[6 + d] = e3[d];
+    }
+}
+
+// One history record after a step (src/solver.rs:114-187): each part's
+// mass, mass-weighted mean displacement, velocity and acceleration and
+// kinetic energy (block sums in a fixed order), and each accelerometer
+// node's position, displacement, velocity and acceleration with its
+// frame's origin and axes.  Every thread of the block calls it.
+template <int B>
+__device__ void history_sample(const InstDev& D, int k, long long step, double tn, double* s_w) {
+#pragma clang fp contract(off)
+    const int na = D.n_groups < 8 ? D.n_groups : 8;
+    double* rec = D.hist + (size_t)k * hist_record_len(D.n_parts, D.n_groups);
+    for (int p = 0; p < D.n_parts; ++p) {
+        double sm[CRUSH_PART_FIELDS];
+#pragma unroll
+        for (int f = 0; f < CRUSH_PART_FIELDS; ++f) sm[f] = 0.0;
+        for (int i = D.p_nodes_ptr[p] + (int)threadIdx.x; i < D.p_nodes_ptr[p + 1]; i += B) {
+            const int n = D.p_nodes[i];
+            const double m = D.mass[n];
+            double a[3];
+            node_accel(D, n, a);
+            sm[0] += m;
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const double vd = D.v[3 * (size_t)n + d];
+                sm[1 + d] += m * D.u[3 * (size_t)n + d];
+                sm[4 + d] += m * vd;
+                sm[7 + d] += m * a[d];
+                sm[10] += 0.5 * m * vd * vd;
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < CRUSH_PART_FIELDS; ++f) sm[f] = block_reduce<B>(sm[f], 0, s_w);
+        if (threadIdx.x == 0) {
+            double* r = rec + 2 + (size_t)CRUSH_PART_FIELDS * p;
+            const double inv = sm[0] > 0.0 ? 1.0 / sm[0] : 0.0;
+            r[0] = sm[0];
+            for (int f = 1; f < 10; ++f) r[f] = sm[f] * inv;
+            r[10] = sm[10];
+        }
+    }
+    if (threadIdx.x == 0) {
+        rec[0] = tn;
+        rec[1] = (double)step;
+        for (int g = 0; g < na; ++g) {
+            double* r = rec + 2 + (size_t)CRUSH_PART_FIELDS * D.n_parts + (size_t)CRUSH_ACCEL_FIELDS * g;
+            const int n = D.g_accel[g];
+            if (n < 0) {
+                for (int f = 0; f < CRUSH_ACCEL_FIELDS; ++f) r[f] = 0.0;
+                r[15] = r[19] = r[23] = 1.0;
+                continue;
+            }
+            node_position(D, n, r);
+            for (int d = 0; d < 3; ++d) {
+                r[3 + d] = D.u[3 * (size_t)n + d];
+                r[6 + d] = D.v[3 * (size_t)n + d];
+            }
+            node_accel(D, n, r + 9);
+            const int o = D.g_frame[3 * g], x = D.g_frame[3 * g + 1], pl = D.g_frame[3 * g + 2];
+            if (o >= 0) {
+                node_position(D, o, r + 12);
+                frame_axes(D, o, x, pl, r + 15);
+            } else {
+                for (int d = 0; d < 3; ++d) r[12 + d] = r[d];
+                for (int f = 0; f < 9; ++f) r[15 + f] = (f == 0 || f == 4 || f == 8) ? 1.0 : 0.0;
+            }
+        }
+    }
+}
+
 // End of a step: time, step count, crush samples, separation and the stop
 // decision (the loop condition of src/solver.rs:287 plus the optional
 // retirement at separation).  c is the block's control block in LDS; only
@@ -786,8 +979,12 @@ __device__ bool step_end(const InstDev& D, StepCtl* c, double dt, int pair_hits,
     }
     const double tn = c->t + dt;
     const bool done = sep || bad || !(tn < c->end_time);
+    const int hk = c->hist_k;
+    const bool hist = D.n_hist > 0 && (step % D.hist_steps == 0 || done);
+    if (hist && hk < D.n_hist) history_sample<B>(D, hk, step, tn, s_w);
     __syncthreads();
     if (threadIdx.x == 0) {
+        if (hist) c->hist_k = hk + 1;
         if (D.n_pulse > 0) pulse_sample(D, c, tn);
         for (int g = 0; g < ng; ++g) c->crush_max[g] = fmax(c->crush_max[g], cr[g]);
         c->had_pair = had_pair;

@@ -577,11 +577,13 @@ void prepare(const crush_model& m, Prepared& p) {
             p.g_front.push_back(n);
         }
         if (gr.accel_node >= N || gr.axis < 0 || gr.axis > 2) fail(CRUSH_E_INPUT, "group %d: bad accelerometer", g);
-        if (gr.frame_origin >= N || gr.frame_x >= N || ((gr.frame_origin < 0) != (gr.frame_x < 0)) ||
+        if (gr.frame_origin >= N || gr.frame_x >= N || gr.frame_plane >= N ||
+            ((gr.frame_origin < 0) != (gr.frame_x < 0)) || (gr.frame_origin < 0 && gr.frame_plane >= 0) ||
             (gr.frame_origin >= 0 && gr.frame_origin == gr.frame_x))
             fail(CRUSH_E_INPUT, "group %d: bad accelerometer frame", g);
         p.g_frame.push_back(gr.frame_origin < 0 ? -1 : gr.frame_origin);
         p.g_frame.push_back(gr.frame_x < 0 ? -1 : gr.frame_x);
+        p.g_frame.push_back(gr.frame_plane < 0 ? -1 : gr.frame_plane);
         p.g_heading.push_back(gr.heading > 0 ? 1 : (gr.heading < 0 ? -1 : 0));
         p.g_nodes_ptr.push_back((int32_t)p.g_nodes.size());
         p.g_front_ptr.push_back((int32_t)p.g_front.size());
@@ -600,6 +602,41 @@ void prepare(const crush_model& m, Prepared& p) {
     } else if (s.pulse_dt < 0.0 || !(s.pulse_dt == s.pulse_dt)) {
         fail(CRUSH_E_INPUT, "settings: pulse_dt must be >= 0");
     }
+    // part histories: the nodes of each part's elements, ascending
+    // (Mesh::part_nodes, src/mesh.rs:74-82)
+    if (s.history_steps < 0 || s.frame_steps < 0) fail(CRUSH_E_INPUT, "settings: history_steps and frame_steps must be >= 0");
+    p.n_parts = m.n_parts;
+    p.p_nodes_ptr.assign(1, 0);
+    p.p_nodes.clear();
+    p.n_hist = 0;
+    if (s.history_steps > 0) {
+        std::vector<std::vector<int32_t>> pn(m.n_parts);
+        for (int e = 0; e < E; ++e)
+            for (int i = 0; i < 8; ++i) pn[m.hex_part[e]].push_back(m.hexes[8 * e + i]);
+        for (auto& v : pn) {
+            std::sort(v.begin(), v.end());
+            v.erase(std::unique(v.begin(), v.end()), v.end());
+            p.p_nodes.insert(p.p_nodes.end(), v.begin(), v.end());
+            p.p_nodes_ptr.push_back((int32_t)p.p_nodes.size());
+        }
+        p.n_hist = history_capacity(p, s.history_steps);
+    }
+}
+
+// Device history samples a solve can take: one per history_steps steps
+// plus the last step's, with the step count bounded by the smallest step
+// the solve can take (the fixed step, or dt_scale x min(dt_floor,
+// contact_dt) under mass scaling); HISTORY_MAX_SAMPLES when unbounded.
+int history_capacity(const Prepared& p, int every) {
+    const crush_settings& s = p.settings;
+    double dt_min = 0.0;
+    if (s.time_step > 0.0 || !p.adaptive)
+        dt_min = p.dt0;
+    else if (p.dt_floor > 0.0)
+        dt_min = s.dt_scale * std::min(p.dt_floor, p.contact_dt);
+    double n = HISTORY_MAX_SAMPLES;
+    if (dt_min > 0.0) n = std::min(n, std::ceil(s.end_time / dt_min) / (double)every + 2.0);
+    return (int)std::max(n, 1.0);
 }
 
 }  // namespace crush

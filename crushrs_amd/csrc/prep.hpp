@@ -74,18 +74,27 @@ struct Prepared {
     // groups (outputs)
     int n_groups = 0;
     std::vector<int32_t> g_nodes_ptr, g_nodes, g_front_ptr, g_front, g_accel, g_axis;
-    // accelerometer frames [2*G] (origin, x-axis node; -1 = global x) and
-    // directions of travel [G]
+    // accelerometer frames [3*G] (origin, x-axis node, plane node; -1 =
+    // none) and directions of travel [G]
     std::vector<int32_t> g_frame, g_heading;
     // accelerometer pulse samples per group (0 = no pulse): t = k pulse_dt
     // for k < n_pulse covers [0, end_time]
     int n_pulse = 0;
+    // part histories: nodes of each part (CSR, ascending) and the device
+    // samples one solve keeps (0 = no history)
+    std::vector<int32_t> p_nodes_ptr, p_nodes;
+    int n_hist = 0;
     // total mass and lumped masses of each group (for outputs)
     std::vector<double> g_mass, g_v0;  // g_v0: [3*G] initial mean velocity
 };
 
 // Validate and prepare; throws crush::Error.
 void prepare(const crush_model& m, Prepared& p);
+
+// Upper bound of the history samples recorded on the device after the
+// initial one (see prepare).
+constexpr int HISTORY_MAX_SAMPLES = 1 << 16;
+int history_capacity(const Prepared& p, int every);
 
 // Element geometry of one hexahedron (product implementation): centre
 // gradient [24], volume, lumped masses [8] (unit density), hourglass base

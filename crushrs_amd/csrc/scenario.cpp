@@ -4,6 +4,7 @@
 #include "scenario.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstring>
 #include <fstream>
@@ -447,6 +448,7 @@ void ModelHolder::flatten() {
         cg.axis = g.axis;
         cg.frame_origin = g.frame_origin;
         cg.frame_x = g.frame_x;
+        cg.frame_plane = g.frame_plane;
         cg.heading = g.heading;
         groups.push_back(cg);
     }
@@ -1020,9 +1022,10 @@ int vehicle_accel_node(const Model& model, const std::string& part, const std::s
 // The frame of add_accelerometer_at (src/model.rs:185-213) at node rec of
 // part p: local x from the origin to the x node, which is rec's neighbour
 // best aligned with +X (within 1.75 x the shortest edge at rec, cosine >
-// 0.5), or else rec itself with its best -X neighbour as the origin.  Only
-// local x enters the pulse, so the frame's plane node is not needed.
-static std::pair<int, int> accelerometer_frame(const Mesh& mesh, int rec, int p) {
+// 0.5), or else rec itself with its best -X neighbour as the origin; the
+// plane node is the origin's neighbour best aligned with +Y (cosine > 0.5),
+// else with -Y, other than the x node.  Returns (origin, x node, plane).
+static std::array<int, 3> accelerometer_frame(const Mesh& mesh, int rec, int p) {
     const double radius = 1.75 * mesh.min_edge_at(rec);
     int origin = rec, x_axis = rec;
     auto fwd = mesh.best_aligned_neighbour(rec, {1.0, 0.0, 0.0}, radius, p, {});
@@ -1033,7 +1036,17 @@ static std::pair<int, int> accelerometer_frame(const Mesh& mesh, int rec, int p)
         if (back.first >= 0) origin = back.first;
     }
     if (origin == x_axis) fail(CRUSH_E_INPUT, "accelerometer: part '%s' has no neighbouring nodes along X", mesh.parts[p].c_str());
-    return {origin, x_axis};
+    int plane = -1;
+    auto up = mesh.best_aligned_neighbour(origin, {0.0, 1.0, 0.0}, radius, p, {x_axis});
+    if (up.first >= 0 && up.second > 0.5) {
+        plane = up.first;
+    } else {
+        auto down = mesh.best_aligned_neighbour(origin, {0.0, -1.0, 0.0}, radius, p, {x_axis});
+        if (down.first < 0)
+            fail(CRUSH_E_INPUT, "accelerometer: part '%s' has no neighbouring nodes to define a frame", mesh.parts[p].c_str());
+        plane = down.first;
+    }
+    return {origin, x_axis, plane};
 }
 
 Group make_group(const Model& model, const std::string& name, const std::string& front_face, int axis,
@@ -1043,8 +1056,9 @@ Group make_group(const Model& model, const std::string& name, const std::string&
     if (!model.mesh.face_set_nodes(front_face, g.front)) fail(CRUSH_E_INPUT, "no face set '%s'", front_face.c_str());
     g.accel_node = vehicle_accel_node(model, name, front_face);
     auto fr = accelerometer_frame(model.mesh, g.accel_node, model.mesh.part_index(name));
-    g.frame_origin = fr.first;
-    g.frame_x = fr.second;
+    g.frame_origin = fr[0];
+    g.frame_x = fr[1];
+    g.frame_plane = fr[2];
     g.axis = axis;
     g.heading = heading;
     return g;

@@ -69,6 +69,7 @@ struct Group {
     int accel_node = -1;
     int axis = 0;
     int frame_origin = -1, frame_x = -1;  // accelerometer frame (local x from origin to x)
+    int frame_plane = -1;                 // the frame's x-y plane node
     int heading = 0;                      // +1 / -1 along global x, 0 unknown
 };
 

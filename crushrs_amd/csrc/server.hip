@@ -23,6 +23,7 @@
 
 #include "common.hpp"
 #include "engine.hpp"
+#include "outputs.hpp"
 #include "signal.hpp"
 
 namespace crush {
@@ -97,7 +98,7 @@ struct Request {
     int status = CRUSH_OK;
     std::string error;
     StepCtl ctl{};
-    std::vector<double> dv, u, v, pulse;
+    std::vector<double> dv, u, v, pulse, hist;
 };
 
 }  // namespace
@@ -195,6 +196,7 @@ static void read_result(ServerImpl& S, Request& r) {
         HIP_OK(hipMemcpy(r.v.data(), base + r.o.v, sizeof(double) * r.v.size(), hipMemcpyDeviceToHost));
     }
     r.pulse = pulse_download(P, reinterpret_cast<const double*>(base + r.o.pulse));
+    r.hist = history_download(P, reinterpret_cast<const double*>(base + r.o.hist), r.ctl.hist_k, E.stream);
     if (r.ctl.error) {
         r.status = CRUSH_E_NONFINITE;
         r.error = "displacement became non-finite at step " + std::to_string(r.ctl.step);
@@ -378,6 +380,8 @@ static void fill(const Request& r, const Prepared& P, crush_results* out) {
             gr.crush_max = g < 8 ? r.ctl.crush_max[g] : 0.0;
         }
     pulse_results_host(P, r.pulse.empty() ? nullptr : r.pulse.data(), r.ctl.pulse_k, r.ctl.t, out);
+    history_results(P, r.hist.data(), r.ctl.hist_k, out);
+    out->frame_samples = 0;
 }
 
 // Hand a finished request to the caller and release its ticket (mu held).

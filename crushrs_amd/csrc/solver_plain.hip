@@ -14,6 +14,7 @@
 #include "common.hpp"
 #include "device_common.hpp"
 #include "hip_util.hpp"
+#include "outputs.hpp"
 #include "phases.hpp"
 #include "prep.hpp"
 #include "signal.hpp"
@@ -35,8 +36,8 @@ struct DeviceModel {
     DevBuf<double> cn_scale, fgeo, rec_f, face_force;
     DevBuf<int32_t> rec_face, fh_count, fh_list;
     DevBuf<int32_t> g_nodes_ptr, g_nodes, g_front_ptr, g_front, g_accel, g_axis;
-    DevBuf<double> g_mass, g_v0, g_dv, pulse;
-    DevBuf<int32_t> g_frame;
+    DevBuf<double> g_mass, g_v0, g_dv, pulse, hist;
+    DevBuf<int32_t> g_frame, p_nodes_ptr, p_nodes;
     DevBuf<StepCtl> ctl;
     InstDev inst{};
 
@@ -111,6 +112,9 @@ struct DeviceModel {
         g_frame.upload(P.g_frame, s);
         // written in order from sample 0 as the steps end
         pulse.alloc((size_t)std::min(P.n_groups, 8) * (size_t)P.n_pulse);
+        p_nodes_ptr.upload(P.p_nodes_ptr, s);
+        p_nodes.upload(P.p_nodes, s);
+        hist.alloc(hist_record_len(P.n_parts, P.n_groups) * (size_t)P.n_hist);
         StepCtl c;
         std::memset(&c, 0, sizeof c);
         c.t = 0.0;
@@ -191,6 +195,12 @@ struct DeviceModel {
         I.pulse = pulse.p;
         I.n_pulse = P.n_pulse;
         I.pulse_dt = P.settings.pulse_dt;
+        I.p_nodes_ptr = p_nodes_ptr.p;
+        I.p_nodes = p_nodes.p;
+        I.hist = hist.p;
+        I.n_parts = P.n_parts;
+        I.n_hist = P.n_hist;
+        I.hist_steps = std::max(P.settings.history_steps, 1);
         I.ctl = ctl.p;
     }
 };
@@ -417,6 +427,8 @@ static void fill_results(const Prepared& P, PlainSolver& S, const StepCtl& c, cr
         }
     }
     pulse_results(P, S.D->pulse.p, c.pulse_k, c.t, out);
+    const std::vector<double> h = history_download(P, S.D->hist.p, c.hist_k, S.stream);
+    history_results(P, h.data(), c.hist_k, out);
 }
 
 // Guard against a solve that never reaches end_time (dt is floored at 1e-15).
@@ -432,12 +444,28 @@ void run_plain(const crush_model& m, crush_results* out) {
     HIP_OK(hipEventCreate(&e1));
     HIP_OK(hipEventRecord(e0, S.stream));
     const int chunk = 32;
+    // frames: the initial state, every frame_steps steps, the last step
+    const int fs = P.settings.frame_steps;
+    FrameStore frames;
+    if (fs > 0 && out) frames.init(P, out->frame_capacity, S.stream);
     StepCtl c;
     while (true) {
-        for (int i = 0; i < chunk; ++i) S.launch_step();
+        for (int i = 0; i < chunk; ++i) {
+            S.launch_step();
+            if (frames.cap > 0 && S.host_step % fs == 0)
+                frames.record(S.D->inst, P, (int)(S.host_step / fs), S.host_step, S.stream);
+        }
         c = S.read_ctl();
         if (c.done) break;
         if (S.host_step > kMaxSteps) fail(CRUSH_E_INPUT, "no end after %lld steps (t = %.3e)", S.host_step, c.t);
+    }
+    long long n_frames = 0;
+    if (fs > 0) {
+        n_frames = 1 + c.step / fs;
+        if (c.step % fs != 0) {
+            if (frames.cap > 0) frames.record(S.D->inst, P, (int)std::min<long long>(n_frames, INT32_MAX), c.step, S.stream);
+            ++n_frames;
+        }
     }
     HIP_OK(hipEventRecord(e1, S.stream));
     HIP_OK(hipEventSynchronize(e1));
@@ -448,6 +476,8 @@ void run_plain(const crush_model& m, crush_results* out) {
     if (c.error) fail(CRUSH_E_NONFINITE, "displacement became non-finite at step %lld (t = %.3e)", c.step, c.t);
     if (out) {
         fill_results(P, S, c, out);
+        if (fs > 0) frames.fill(P, (int)std::min<long long>(n_frames, INT32_MAX), out, S.stream);
+        else out->frame_samples = 0;
         out->device_ms = ms;
         out->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }

@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define CRUSH_ABI_VERSION 2
+#define CRUSH_ABI_VERSION 3
 
 /* Status codes.  The reference panics in all of these cases
  * (src/solver.rs:302 non-finite displacement, src/model.rs:186-242 unknown
@@ -80,8 +80,8 @@ typedef struct crush_contact {
     int32_t reserved0;
 } crush_contact;
 
-/* src/model.rs:75-150 (Settings).  Fields the GPU core does not use
- * (frame_steps, history_steps, simd, threads) belong to the output layer. */
+/* src/model.rs:75-150 (Settings).  simd and threads select CPU code paths
+ * of the reference and have no counterpart here. */
 typedef struct crush_settings {
     double end_time;             /* s (reference default 0.15)                  */
     double dt_scale;             /* safety factor on the stable step (0.85)     */
@@ -104,6 +104,14 @@ typedef struct crush_settings {
      * history_steps = 1 with Pulse::from_history at PULSE_DT = 1e-4 s).
      * 0 = off. */
     double pulse_dt;
+    /* Part and accelerometer histories (src/solver.rs:114-187, 276-282,
+     * 349-357): a sample of the initial state, one after every
+     * history_steps-th step and one after the last step.  0 = off. */
+    int32_t history_steps;
+    /* Deformed-mesh frames (src/solver.rs:265-277, 346-355): the initial
+     * state, after every frame_steps-th step and after the last step.
+     * crush_run only (batches and servers ignore it).  0 = off. */
+    int32_t frame_steps;
 } crush_settings;
 
 /* A vehicle, for the delta-v / crush outputs of src/bin/crushrs.rs:237-242
@@ -123,7 +131,9 @@ typedef struct crush_group {
     /* direction of travel along global x (+1 / -1; 0 = unknown): the pulse
      * delta-v is reported in that direction (src/headon.rs:70-78) */
     int32_t heading;
-    int32_t reserved0;
+    /* the frame's plane node (local x-y plane, +y side), -1 = none: local
+     * y and z of the accelerometer history (src/model.rs:30-58) */
+    int32_t frame_plane;
 } crush_group;
 
 /* src/model.rs:152-165 (Model) + src/mesh.rs:37-52 (Mesh), flattened. */
@@ -183,7 +193,41 @@ typedef struct crush_results {
     double* pulse;
     int32_t pulse_capacity;
     int32_t pulse_samples;
+    /* Histories (settings.history_steps > 0; src/solver.rs:19-62, 114-187).
+     * Sample k < min(history_samples, history_capacity) was taken after
+     * step history_step[k] at time history_time[k]; sample 0 is the initial
+     * state.  history[(k * n_parts + p) * CRUSH_PART_FIELDS + f] is part
+     * p's state (PartState): mass, mass-weighted mean displacement,
+     * velocity and acceleration (xyz each) and kinetic energy.
+     * accel_history[(k * n_accel + g) * CRUSH_ACCEL_FIELDS + f], n_accel =
+     * min(n_groups, 8), is group g's accelerometer node (NodeHistory and
+     * FrameHistory): position, displacement, velocity, acceleration (xyz
+     * each, global axes), the frame origin's position and the frame's
+     * local x, y, z unit axes in global components (the identity when the
+     * group has no frame).  Any pointer may be NULL. */
+    double* history_time;
+    int64_t* history_step;
+    double* history;
+    double* accel_history;
+    int32_t history_capacity;
+    int32_t history_samples;
+    /* Frames (settings.frame_steps > 0, crush_run only; Frame,
+     * src/solver.rs:8-19): frame k < min(frame_samples, frame_capacity) at
+     * frame_time[k]: nodal displacement [k][3 * n_nodes], and per element
+     * in model order the equivalent plastic strain / compaction [k][n_hexes],
+     * the Cauchy stress in Voigt order xx yy zz xy yz xz [k][6 * n_hexes]
+     * and the eroded flag [k][n_hexes].  Any pointer may be NULL. */
+    double* frame_time;
+    float* frame_displacement;
+    float* frame_plastic_strain;
+    float* frame_stress;
+    uint8_t* frame_eroded;
+    int32_t frame_capacity;
+    int32_t frame_samples;
 } crush_results;
+
+#define CRUSH_PART_FIELDS 11
+#define CRUSH_ACCEL_FIELDS 24
 
 /* ---- core entry points ------------------------------------------------- */
 
@@ -218,6 +262,12 @@ int32_t crush_debug_run_steps(const crush_model* model, int32_t n_steps,
 /* Number of contacting nodes the static broad phase keeps, summed over the
  * contacts (the row length of crush_debug_run_steps' hit). */
 int32_t crush_contact_node_count(const crush_model* model, int32_t* count);
+
+/* Sizes a caller allocates for the histories and frames of a solve of this
+ * model: the largest history_samples and frame_samples it can report
+ * (each 0 when the settings ask for none).  No counterpart in the
+ * reference, whose Results grow as the solve runs. */
+int32_t crush_output_capacity(const crush_model* model, int32_t* history_samples, int32_t* frame_samples);
 
 
 /* ---- batched solves ----------------------------------------------------

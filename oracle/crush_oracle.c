@@ -1185,6 +1185,224 @@ static double accel_local_x(const crush_model* m, const crush_group* g, const do
     return e[0] * a[0] + e[1] * a[1] + e[2] * a[2];
 }
 
+/* ------------------------------------------------ histories and frames */
+
+/* Growable record store of the histories: per sample the time, the step,
+ * part_states (src/solver.rs:114-144) and sample_accelerometers
+ * (src/solver.rs:150-187). */
+typedef struct {
+    int n_parts, n_acc;
+    int* part_ptr; /* [n_parts + 1] */
+    int* part_nodes;
+    size_t n, cap, rec;
+    double* data;
+} history_t;
+
+static int cmp_int(const void* a, const void* b) {
+    int x = *(const int*)a, y = *(const int*)b;
+    return (x > y) - (x < y);
+}
+
+/* Mesh::part_nodes (src/mesh.rs:74-82): nodes of each part's elements,
+ * ascending, without repeats. */
+static void history_init(const crush_model* m, history_t* h) {
+    memset(h, 0, sizeof *h);
+    h->n_parts = m->n_parts;
+    h->n_acc = m->n_groups < 8 ? m->n_groups : 8;
+    h->rec = 2 + (size_t)CRUSH_PART_FIELDS * h->n_parts + (size_t)CRUSH_ACCEL_FIELDS * h->n_acc;
+    h->part_ptr = (int*)calloc((size_t)m->n_parts + 1, sizeof(int));
+    h->part_nodes = (int*)malloc(sizeof(int) * (8 * (size_t)m->n_hexes + 1));
+    int* tmp = (int*)malloc(sizeof(int) * (8 * (size_t)m->n_hexes + 1));
+    int total = 0;
+    for (int p = 0; p < m->n_parts; ++p) {
+        int k = 0;
+        for (int e = 0; e < m->n_hexes; ++e)
+            if (m->hex_part[e] == p)
+                for (int i = 0; i < 8; ++i) tmp[k++] = m->hexes[8 * e + i];
+        qsort(tmp, (size_t)k, sizeof(int), cmp_int);
+        for (int i = 0; i < k; ++i)
+            if (i == 0 || tmp[i] != tmp[i - 1]) h->part_nodes[total++] = tmp[i];
+        h->part_ptr[p + 1] = total;
+    }
+    free(tmp);
+    h->cap = 256;
+    h->data = (double*)malloc(sizeof(double) * h->rec * h->cap);
+}
+
+static void history_free(history_t* h) {
+    free(h->part_ptr);
+    free(h->part_nodes);
+    free(h->data);
+}
+
+/* LocalFrame::axes (src/model.rs:44-58), rows e1, e2, e3. */
+static void frame_axes(const double* xo, const double* xx, const double* xp, double* ax) {
+    double e1[3], p[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < 3; ++d) e1[d] = xx[d] - xo[d];
+    if (xp)
+        for (int d = 0; d < 3; ++d) p[d] = xp[d] - xo[d];
+    double n1 = sqrt(dot(e1, e1));
+    if (n1 < 1e-300) {
+        e1[0] = 1.0;
+        e1[1] = e1[2] = 0.0;
+        n1 = 1.0;
+    }
+    for (int d = 0; d < 3; ++d) e1[d] /= n1;
+    double c[3];
+    cross(p, e1, c);
+    if (sqrt(dot(c, c)) < 1e-12 * fmax(sqrt(dot(p, p)), 1e-300)) {
+        int xa = fabs(e1[0]) < 0.9;
+        p[0] = xa ? 1.0 : 0.0;
+        p[1] = xa ? 0.0 : 1.0;
+        p[2] = 0.0;
+    }
+    double e3[3], e2[3];
+    cross(e1, p, e3);
+    double n3 = sqrt(dot(e3, e3));
+    for (int d = 0; d < 3; ++d) e3[d] /= n3;
+    cross(e3, e1, e2);
+    for (int d = 0; d < 3; ++d) {
+        ax[d] = e1[d];
+        ax[3 + d] = e2[d];
+        ax[6 + d] = e3[d];
+    }
+}
+
+static void history_record(const crush_model* m, history_t* h, double t, long step, const double* masses,
+                           const double* u, const double* v, const double* acc) {
+    if (h->n == h->cap) {
+        h->cap *= 2;
+        h->data = (double*)realloc(h->data, sizeof(double) * h->rec * h->cap);
+    }
+    double* r = h->data + h->n * h->rec;
+    r[0] = t;
+    r[1] = (double)step;
+    for (int p = 0; p < h->n_parts; ++p) {
+        double mt = 0.0, mv[3] = {0, 0, 0}, mu[3] = {0, 0, 0}, ma[3] = {0, 0, 0}, ke = 0.0;
+        for (int i = h->part_ptr[p]; i < h->part_ptr[p + 1]; ++i) {
+            int n = h->part_nodes[i];
+            mt += masses[n];
+            for (int d = 0; d < 3; ++d) {
+                mv[d] += masses[n] * v[3 * n + d];
+                mu[d] += masses[n] * u[3 * n + d];
+                ma[d] += masses[n] * acc[3 * n + d];
+                ke += 0.5 * masses[n] * v[3 * n + d] * v[3 * n + d];
+            }
+        }
+        double inv = mt > 0.0 ? 1.0 / mt : 0.0;
+        double* q = r + 2 + (size_t)CRUSH_PART_FIELDS * p;
+        q[0] = mt;
+        for (int d = 0; d < 3; ++d) {
+            q[1 + d] = mu[d] * inv;
+            q[4 + d] = mv[d] * inv;
+            q[7 + d] = ma[d] * inv;
+        }
+        q[10] = ke;
+    }
+    for (int g = 0; g < h->n_acc; ++g) {
+        const crush_group* gr = &m->groups[g];
+        double* q = r + 2 + (size_t)CRUSH_PART_FIELDS * h->n_parts + (size_t)CRUSH_ACCEL_FIELDS * g;
+        memset(q, 0, sizeof(double) * CRUSH_ACCEL_FIELDS);
+        int n = gr->accel_node;
+        if (n < 0) {
+            q[15] = q[19] = q[23] = 1.0;
+            continue;
+        }
+        for (int d = 0; d < 3; ++d) {
+            q[d] = m->nodes[3 * n + d] + u[3 * n + d];
+            q[3 + d] = u[3 * n + d];
+            q[6 + d] = v[3 * n + d];
+            q[9 + d] = acc[3 * n + d];
+        }
+        if (gr->frame_origin >= 0) {
+            double xo[3], xx[3], xp[3];
+            for (int d = 0; d < 3; ++d) {
+                xo[d] = m->nodes[3 * gr->frame_origin + d] + u[3 * gr->frame_origin + d];
+                xx[d] = m->nodes[3 * gr->frame_x + d] + u[3 * gr->frame_x + d];
+                if (gr->frame_plane >= 0) xp[d] = m->nodes[3 * gr->frame_plane + d] + u[3 * gr->frame_plane + d];
+            }
+            for (int d = 0; d < 3; ++d) q[12 + d] = xo[d];
+            frame_axes(xo, xx, gr->frame_plane >= 0 ? xp : NULL, q + 15);
+        } else {
+            for (int d = 0; d < 3; ++d) q[12 + d] = q[d];
+            q[15] = q[19] = q[23] = 1.0;
+        }
+    }
+    ++h->n;
+}
+
+/* Frame capture (src/solver.rs:265-275): displacement, plastic strain
+ * (compaction) and Cauchy stress (Elements::cauchy_stress,
+ * src/kernel.rs:281-320) per element, eroded flags. */
+typedef struct {
+    int cap, n;
+    double* time;
+    float *u, *plastic, *stress;
+    uint8_t* eroded;
+} frames_t;
+
+static void frame_capture(const crush_model* m, const elements_t* el, const double* u, double t, frames_t* f) {
+    if (f->n >= f->cap) {
+        ++f->n;
+        return;
+    }
+    int k = f->n++;
+    size_t n3 = 3 * (size_t)m->n_nodes, ne = (size_t)m->n_hexes;
+    f->time[k] = t;
+    for (size_t i = 0; i < n3; ++i) f->u[k * n3 + i] = (float)u[i];
+    /* the lane path keeps honeycomb states in its blocks (sync_states,
+     * src/kernel.rs:261-267) */
+    int* lane_of = NULL;
+    if (el->lanes) {
+        lane_of = (int*)malloc(sizeof(int) * (ne + 1));
+        for (size_t e = 0; e < ne; ++e) lane_of[e] = -1;
+        for (int q = 0; q < el->n_lane; ++q) lane_of[el->lane_ids[q]] = q;
+    }
+    for (int e = 0; e < m->n_hexes; ++e) {
+        double st[25];
+        const double* s = el->state + 25 * (size_t)e;
+        if (lane_of && lane_of[e] >= 0) {
+            lane_blocks_state(el->lanes, lane_of[e], st);
+            s = st;
+        }
+        float* sig = f->stress + (k * ne + (size_t)e) * 6;
+        f->plastic[k * ne + (size_t)e] = el->kind[e] == CRUSH_MAT_HONEYCOMB ? (float)s[24] : 0.0f;
+        f->eroded[k * ne + (size_t)e] = el->eroded[e];
+        for (int q = 0; q < 6; ++q) sig[q] = 0.0f;
+        if (el->eroded[e]) continue;
+        double c[9];
+        if (el->kind[e] == CRUSH_MAT_HONEYCOMB) {
+            const double* r = s + 9;
+            const double* sg = s + 18;
+            double sh[9] = {sg[0], sg[3], sg[5], sg[3], sg[1], sg[4], sg[5], sg[4], sg[2]};
+            double rt[9], rs[9];
+            transpose3(r, rt);
+            mul3(r, sh, rs);
+            mul3(rs, rt, c);
+        } else {
+            double F[9] = {0};
+            const int32_t* hx = m->hexes + 8 * (size_t)e;
+            for (int i = 0; i < 8; ++i)
+                for (int a = 0; a < 3; ++a)
+                    for (int b = 0; b < 3; ++b)
+                        F[3 * a + b] += (m->nodes[3 * hx[i] + a] + u[3 * hx[i] + a]) * el->grad[24 * (size_t)e + 3 * i + b];
+            double j = det3(F);
+            if (j <= 0.0) continue;
+            const crush_material* mat = &m->materials[m->hex_part[e]];
+            double tau[9];
+            oracle_hencky_elastic(mat->youngs_modulus, mat->poisson_ratio, F, tau);
+            for (int q = 0; q < 9; ++q) c[q] = tau[q] / j;
+        }
+        sig[0] = (float)c[0];
+        sig[1] = (float)c[4];
+        sig[2] = (float)c[8];
+        sig[3] = (float)c[1];
+        sig[4] = (float)c[5];
+        sig[5] = (float)c[2];
+    }
+    free(lane_of);
+}
+
 /* solver::run: src/solver.rs:190-381; simd selects the element path
  * (Settings.simd, src/model.rs:100-102). */
 static int solve(const crush_model* m, crush_results* out, taps_t* taps, int simd, int threads) {
@@ -1275,6 +1493,28 @@ static int solve(const crush_model* m, crush_results* out, taps_t* taps, int sim
         for (int g = 0; g < ng_p; ++g) h_a[g] = accel_local_x(m, &m->groups[g], u, f_ext, f_int, inv_mass, is_fixed);
         h_n = 1;
     }
+    /* histories (history_steps) and frames (frame_steps) */
+    double* acc = (double*)calloc((size_t)n + 1, sizeof(double));
+    for (int k = 0; k < n; ++k) acc[k] = is_fixed[k / 3] ? 0.0 : (f_ext[k] - f_int[k]) * inv_mass[k / 3];
+    history_t hist;
+    int hs = s->history_steps > 0 ? s->history_steps : 0;
+    if (hs) {
+        history_init(m, &hist);
+        history_record(m, &hist, 0.0, 0, masses, u, v, acc);
+    }
+    int fs = s->frame_steps > 0 ? s->frame_steps : 0;
+    frames_t frames;
+    memset(&frames, 0, sizeof frames);
+    if (fs && out && out->frame_capacity > 0) {
+        frames.cap = out->frame_capacity;
+        size_t n3 = (size_t)n, ne = (size_t)m->n_hexes, fc = (size_t)frames.cap;
+        frames.time = (double*)calloc(fc, sizeof(double));
+        frames.u = (float*)calloc(fc * n3, sizeof(float));
+        frames.plastic = (float*)calloc(fc * ne, sizeof(float));
+        frames.stress = (float*)calloc(fc * ne * 6, sizeof(float));
+        frames.eroded = (uint8_t*)calloc(fc * ne, 1);
+    }
+    if (fs) frame_capture(m, &el, u, 0.0, &frames);
 
     double t = 0.0;
     long step = 0;
@@ -1373,6 +1613,11 @@ static int solve(const crush_model* m, crush_results* out, taps_t* taps, int sim
                 h_a[8 * h_n + g] = accel_local_x(m, &m->groups[g], u, f_ext, f_int, inv_mass, is_fixed);
             ++h_n;
         }
+        if (hs || fs) {
+            for (int k = 0; k < n; ++k) acc[k] = is_fixed[k / 3] ? 0.0 : (f_ext[k] - f_int[k]) * inv_mass[k / 3];
+            if (hs && step % hs == 0) history_record(m, &hist, t, step, masses, u, v, acc);
+            if (fs && step % fs == 0) frame_capture(m, &el, u, t, &frames);
+        }
         for (int g = 0; g < m->n_groups && g < 8; ++g)
             crush_max[g] = fmax(crush_max[g], group_crush(&m->groups[g], u));
         /* retirement at separation (crushrs.h, crush_settings) */
@@ -1397,6 +1642,37 @@ static int solve(const crush_model* m, crush_results* out, taps_t* taps, int sim
     }
 done:
     if (taps) taps->steps_done = (int32_t)step;
+    if (rc == CRUSH_OK) {
+        if (hs && step % hs != 0) history_record(m, &hist, t, step, masses, u, v, acc);
+        if (fs && step % fs != 0) frame_capture(m, &el, u, t, &frames);
+    }
+    if (out) {
+        out->history_samples = hs ? (int32_t)hist.n : 0;
+        for (size_t k = 0; hs && k < hist.n && (int32_t)k < out->history_capacity; ++k) {
+            const double* r = hist.data + k * hist.rec;
+            if (out->history_time) out->history_time[k] = r[0];
+            if (out->history_step) out->history_step[k] = (int64_t)r[1];
+            if (out->history)
+                memcpy(out->history + k * (size_t)hist.n_parts * CRUSH_PART_FIELDS, r + 2,
+                       sizeof(double) * CRUSH_PART_FIELDS * (size_t)hist.n_parts);
+            if (out->accel_history && hist.n_acc > 0)
+                memcpy(out->accel_history + k * (size_t)hist.n_acc * CRUSH_ACCEL_FIELDS,
+                       r + 2 + (size_t)CRUSH_PART_FIELDS * hist.n_parts, sizeof(double) * CRUSH_ACCEL_FIELDS * (size_t)hist.n_acc);
+        }
+        out->frame_samples = fs ? frames.n : 0;
+        int fk = frames.n < frames.cap ? frames.n : frames.cap;
+        size_t n3 = (size_t)n, ne = (size_t)m->n_hexes;
+        if (fk > 0) {
+            if (out->frame_time) memcpy(out->frame_time, frames.time, sizeof(double) * (size_t)fk);
+            if (out->frame_displacement) memcpy(out->frame_displacement, frames.u, sizeof(float) * n3 * (size_t)fk);
+            if (out->frame_plastic_strain) memcpy(out->frame_plastic_strain, frames.plastic, sizeof(float) * ne * (size_t)fk);
+            if (out->frame_stress) memcpy(out->frame_stress, frames.stress, sizeof(float) * 6 * ne * (size_t)fk);
+            if (out->frame_eroded) memcpy(out->frame_eroded, frames.eroded, ne * (size_t)fk);
+        }
+    }
+    if (hs) history_free(&hist);
+    free(frames.time); free(frames.u); free(frames.plastic); free(frames.stress); free(frames.eroded);
+    free(acc);
     if (out) {
         out->time = t;
         out->steps = step;

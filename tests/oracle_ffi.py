@@ -94,7 +94,7 @@ def ptr(a: np.ndarray):
 def run(model):
     """oracle_run on a crushrs_amd.Model -> crushrs_amd.Results."""
     import crushrs_amd as ca
-    r, u, v, m, g = ca.alloc_results(model.n_nodes, model.view.n_groups)
+    r, u, v, m, g = ca.alloc_model_results(model)
     rc = load().oracle_run(model.ptr, C.byref(r))
     assert rc == 0, f"oracle_run failed: {rc}"
     return ca.to_results(r, u, v, m, g, model.settings.pulse_dt)
@@ -103,7 +103,7 @@ def run(model):
 def run_lanes(model, threads: int = 1):
     """oracle_run_lanes (the reference's default f32 AVX2 element path)."""
     import crushrs_amd as ca
-    r, u, v, m, g = ca.alloc_results(model.n_nodes, model.view.n_groups)
+    r, u, v, m, g = ca.alloc_model_results(model)
     rc = load().oracle_run_lanes(model.ptr, threads, C.byref(r))
     assert rc == 0, f"oracle_run_lanes failed: {rc}"
     return ca.to_results(r, u, v, m, g, model.settings.pulse_dt)

@@ -30,6 +30,6 @@ def test_library_exports_every_declared_symbol():
 
 def test_abi_version_and_error_string():
     lib = _abi.load()
-    assert lib.crush_abi_version() == 2
+    assert lib.crush_abi_version() == 3
     assert isinstance(lib.crush_last_error(), bytes)
     assert lib.crush_device_count() >= 0

@@ -53,7 +53,9 @@ def test_calibrate_pulse_batched_equals_plain_and_lowers_the_objective():
     assert fb.history == fp.history
     assert fb.start_objective == fp.start_objective and fb.objective == fp.objective
     assert np.array_equal(cal.force_table(fb.vehicle), cal.force_table(fp.vehicle))
-    assert fb.batches == 2 and fb.evaluations == fp.evaluations >= 19 + 17
+    # probes the non-decreasing rule turns into no-ops are skipped (the
+    # measured-curve start has equal knots)
+    assert fb.batches == 2 and fb.evaluations == fp.evaluations > 19
     assert fb.objective < fb.start_objective
     assert fb.history[0][3] is not None  # the first round improves on the measured-curve start
 

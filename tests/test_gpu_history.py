@@ -1,0 +1,100 @@
+"""Histories and frames recorded on the GPU (settings.history_steps,
+settings.frame_steps) against the oracle's (src/solver.rs:114-187,
+265-357), and the batched and served histories against the one-model
+path's."""
+import numpy as np
+import pytest
+
+import crushrs_amd as ca
+from crushrs_amd import output
+import oracle_ffi as of
+
+pytestmark = pytest.mark.gpu
+
+
+def crash(end_time=0.03, history_steps=5, frame_steps=0):
+    return ca.Model.crash(30.0, 30.0).set(end_time=end_time, history_steps=history_steps, frame_steps=frame_steps)
+
+
+def common_samples(g, o):
+    """Indices of the samples both solves took (same step number)."""
+    steps = np.intersect1d(g.history_step, o.history_step)
+    gi = np.searchsorted(g.history_step, steps)
+    oi = np.searchsorted(o.history_step, steps)
+    return gi, oi
+
+
+def test_history_matches_oracle_and_conserves_momentum():
+    """Part states and accelerometer samples of crash 30 30 over 0.03 s every
+    5 steps: the GPU's samples at the oracle's steps, part masses to
+    round-off, mean displacements within 1 mm, mean velocities within
+    0.02 m/s, kinetic energies within 0.1 %, frame axes within 1e-4; the
+    GPU's own samples conserve momentum (no road or wall) and end on its
+    final state."""
+    print("case: crash 30 30, 0.03 s, history every 5 steps", flush=True)
+    m = crash()
+    g, o = ca.run(m), of.run(m)
+    assert g.history_step[0] == 0 and g.history_step[-1] == g.steps
+    gi, oi = common_samples(g, o)
+    assert len(gi) >= 0.9 * len(o.history_step)
+    assert np.allclose(g.history_time[gi], o.history_time[oi], rtol=1e-6, atol=0)
+    G, O = g.parts[gi], o.parts[oi]
+    assert np.allclose(G[..., 0], O[..., 0], rtol=1e-12, atol=0)
+    assert np.abs(G[..., 1:4] - O[..., 1:4]).max() < 1e-3
+    assert np.abs(G[..., 4:7] - O[..., 4:7]).max() < 0.02
+    assert np.allclose(G[..., 10], O[..., 10], rtol=1e-3)
+    A, B = g.accel[gi], o.accel[oi]
+    assert np.abs(A[..., 0:6] - B[..., 0:6]).max() < 1e-3
+    assert np.abs(A[..., 6:9] - B[..., 6:9]).max() < 0.05
+    assert np.abs(A[..., 15:24] - B[..., 15:24]).max() < 1e-4
+    P = g.parts
+    mom = (P[:, :, 0:1] * P[:, :, 4:7]).sum(axis=1)
+    p0 = (g.masses[:, None] * m.initial_velocity()).sum(axis=0)
+    assert np.abs(mom - p0).max() < 1e-9 * np.abs(p0).max()
+    hexes = m.hexes()
+    hp = np.ctypeslib.as_array(m.view.hex_part, shape=(m.n_hexes,))
+    for p in range(m.view.n_parts):
+        nodes = np.unique(hexes[hp == p].ravel())
+        mass = g.masses[nodes].sum()
+        v = (g.masses[nodes, None] * g.velocity[nodes]).sum(0) / mass
+        assert np.allclose(P[-1, p, 4:7], v, rtol=0, atol=1e-12 * max(1.0, np.abs(v).max()))
+
+
+def test_frames_match_oracle():
+    """Frames of crash 30 30 every 25 steps: the same frame times, nodal
+    displacements within 2 mm, identical erosion, compaction within 0.01,
+    and element stresses within 2 % of the frame's stress norm."""
+    print("case: crash 30 30, 0.03 s, frames every 25 steps", flush=True)
+    m = crash(history_steps=0, frame_steps=25)
+    g, o = ca.run(m), of.run(m)
+    n = min(len(g.frame_time), len(o.frame_time))
+    assert abs(len(g.frame_time) - len(o.frame_time)) <= 1 and n >= 3
+    k = n - 1 if len(g.frame_time) == len(o.frame_time) else n - 2
+    assert np.allclose(g.frame_time[:k + 1], o.frame_time[:k + 1], rtol=1e-6)
+    assert np.abs(g.frame_displacement[:k + 1] - o.frame_displacement[:k + 1]).max() < 2e-3
+    assert np.array_equal(g.frame_eroded[:k + 1], o.frame_eroded[:k + 1])
+    assert np.abs(g.frame_plastic_strain[:k + 1] - o.frame_plastic_strain[:k + 1]).max() < 1e-2
+    for j in range(1, k + 1):
+        gs, os_ = g.frame_stress[j].astype(float), o.frame_stress[j].astype(float)
+        assert np.linalg.norm(gs - os_) < 2e-2 * np.linalg.norm(os_), j
+
+
+def test_batch_and_server_histories_equal_plain(tmp_path):
+    """Histories of batched and served solves are bitwise the one-model
+    path's, and the Parquet tables write from any of them."""
+    print("case: batch and server histories", flush=True)
+    models = [crash(0.02, 3), ca.Model.barrier(ca.vehicle("neon-pulse"), end_time=0.02).set(history_steps=2)]
+    plain = [ca.run(m) for m in models]
+    b = ca.Batch(models, steps_per_graph=8)
+    b.run()
+    srv = ca.Server(hbm_budget_bytes=2 ** 30, steps_per_graph=8)
+    tickets = [srv.submit(m) for m in models]
+    served = [srv.wait(t) for t in tickets]
+    srv.close()
+    for p, r in [(p, b.result(i)) for i, p in enumerate(plain)] + list(zip(plain, served)):
+        assert np.array_equal(p.history_time, r.history_time)
+        assert np.array_equal(p.history_step, r.history_step)
+        assert np.array_equal(p.parts, r.parts)
+        assert np.array_equal(p.accel, r.accel)
+    paths = output.write_history(models[1], b.result(1), tmp_path / "barrier")
+    assert all(p.exists() for p in paths)
