@@ -592,8 +592,9 @@ def bench_pulse_fit(args):
     fit = cal.calibrate_pulse(v, meas, speed, rounds=args.steps, solve=timed_solve)
     wall = time.perf_counter() - t0
     value = solves[0] / wall
-    probe_models = [ca.Model.barrier(cal.pulse_candidate(fit.vehicle, p, 1.1) or fit.vehicle, speed=speed)
-                    for p in range(cal.N_KNOTS + 2)]
+    # one round's probes of the fitted vehicle (each parameter up and down)
+    probe_models = [ca.Model.barrier(cal.pulse_candidate(fit.vehicle, p, f) or fit.vehicle, speed=speed)
+                    for p in range(cal.N_KNOTS + 2) for f in (1.1, 1.0 / 1.1)]
     steps = ca.run(probe_models[0]).steps
     out = {
         "metric": METRIC,

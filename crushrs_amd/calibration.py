@@ -464,3 +464,206 @@ def calibrate_rail_boxes(a: _abi.Vehicle, b: _abi.Vehicle, setup: HeadOnSetup, m
     va, vb = with_u(pop[best])
     j, cmp_a, cmp_b = scored[best]
     return RailBoxFit(va, vb, j, cmp_a, cmp_b, evaluations, batches, history)
+
+
+# ---------------------------------------------------------------- barrier metrics and calibrate
+
+BARRIER_END = 0.14  # end time of calibrate's barrier runs (src/vehicle.rs:1017)
+
+
+@dataclass
+class BarrierMetrics:
+    """BarrierMetrics (src/vehicle.rs:866-881): energy-equivalent stiffness
+    over the crush window (KW400 for 25-400 mm), maximum crush, peak and mean
+    force of the loading phase, restitution, delta-v, and the (crush, force)
+    samples of the loading phase."""
+    kw400: float
+    max_crush: float
+    peak_force: float
+    mean_force: float
+    restitution: float
+    delta_v: float
+    curve: np.ndarray
+
+
+def vehicle_parts(model: Model, name: str) -> list[int]:
+    """The parts a vehicle was built as: <name>, <name>_crush, <name>_rails,
+    <name>_bumper, <name>_bumper_rails, those that exist
+    (src/vehicle.rs:890-891)."""
+    names = model.part_names()
+    out = [names.index(name + s) for s in ("", "_crush", "_rails", "_bumper", "_bumper_rails") if name + s in names]
+    if not out:
+        raise ValueError(f"no part '{name}'")
+    return out
+
+
+def _energy_at(curve: np.ndarray, d: float) -> float:
+    e = 0.0
+    for (c0, f0), (c1, f1) in zip(curve[:-1], curve[1:]):
+        if c1 <= c0:
+            continue
+        if c1 <= d:
+            e += 0.5 * (f0 + f1) * (c1 - c0)
+        elif c0 < d:
+            f_d = f0 + (f1 - f0) * (d - c0) / (c1 - c0)
+            e += 0.5 * (f0 + f_d) * (d - c0)
+            break
+        else:
+            break
+    return e
+
+
+def barrier_metrics(model: Model, results, name: str, axis: int = 0, window=(0.025, 0.4)) -> BarrierMetrics:
+    """barrier_metrics_window (src/vehicle.rs:884-947) from the part history
+    of a barrier solve (settings.history_steps > 0): the vehicle's
+    mass-weighted mean velocity and displacement along `axis`, force
+    -m dv/dt by central differences smoothed (1, 2, 1) / 4, contact where
+    the force first exceeds 2 % of its peak, crush from there, the loading
+    phase up to the last maximum of crush."""
+    if results.parts is None:
+        raise ValueError("the solve recorded no history (settings.history_steps = 0)")
+    parts = vehicle_parts(model, name)
+    P = results.parts[:, parts, :]
+    m = P[:, :, 0].sum(axis=1)
+    v = (P[:, :, 0] * P[:, :, 4 + axis]).sum(axis=1) / m
+    u = (P[:, :, 0] * P[:, :, 1 + axis]).sum(axis=1) / m
+    t = results.history_time
+    n = len(t)
+    if n <= 3:
+        raise ValueError(f"no history for '{name}'")
+    v0 = v[0]
+    sign = math.copysign(1.0, v0)
+    force = np.zeros(n)
+    for i in range(1, n - 1):
+        force[i] = -sign * m[i - 1] * (v[i + 1] - v[i - 1]) / (t[i + 1] - t[i - 1])
+    sm = force.copy()
+    for i in range(1, n - 1):
+        sm[i] = (force[i - 1] + 2.0 * force[i] + force[i + 1]) / 4.0
+    force = sm
+    peak = max(0.0, float(force.max()))
+    above = np.nonzero(force > 0.02 * peak)[0]
+    contact = int(above[0]) if len(above) else 0
+    crush = sign * (u - u[contact])
+    i_max = n - 1 - int(np.argmax(crush[::-1]))
+    max_crush = float(crush[i_max])
+    curve = np.stack([crush[contact:i_max + 1], force[contact:i_max + 1]], axis=1)
+    w0, w1 = window
+    kw = 2.0 * (_energy_at(curve, w1) - _energy_at(curve, w0)) / (w1 * w1 - w0 * w0)
+    mean = _energy_at(curve, max_crush) / max_crush if max_crush > 0.0 else 0.0
+    v_end = float(v[-1])
+    return BarrierMetrics(kw, max_crush, peak, mean, -v_end / v0, v_end - v0, curve)
+
+
+def kw_window(v: _abi.Vehicle, window=(0.025, 0.4)) -> float:
+    """CrushCurve::kw_window (src/vehicle.rs:60-64) of the bilinear curve."""
+    w0, w1 = window
+    return v.curve[1] + 2.0 * (w1 - w0) / (w1 * w1 - w0 * w0) * v.curve[0]
+
+
+def barrier_prediction(v: _abi.Vehicle, speed: float) -> tuple[float, float]:
+    """Vehicle::barrier_prediction (src/vehicle.rs:518-521): the crush that
+    absorbs 1/2 m v^2 on the bilinear curve, and the force there."""
+    a, b = 0.5 * v.curve[1], v.curve[0]
+    e = 0.5 * v.mass * speed * speed
+    c = (-b + math.sqrt(b * b + 4.0 * a * e)) / (2.0 * a)
+    return c, v.curve[0] + v.curve[1] * c
+
+
+@dataclass
+class BarrierTargets:
+    """BarrierTargets (src/vehicle.rs:983-1004)."""
+    kw400: float
+    max_crush: float
+    restitution: float
+    speed: float
+    window: tuple = (0.025, 0.4)
+
+    @classmethod
+    def from_curve(cls, v: _abi.Vehicle, speed: float, window=(0.025, 0.4), restitution: float = 0.12):
+        return cls(kw_window(v, window), barrier_prediction(v, speed)[0], restitution, speed, tuple(window))
+
+    @classmethod
+    def from_vehicle(cls, v: _abi.Vehicle, restitution: float = 0.12):
+        return cls.from_curve(v, 35.0 * 0.44704, (0.025, 0.4), restitution)
+
+
+@dataclass
+class BarrierFit:
+    """What calibrate returns: the tuned vehicle, the metrics of the last
+    evaluated vehicle, and one (kw400, max_crush, restitution) per
+    iteration."""
+    vehicle: _abi.Vehicle
+    metrics: BarrierMetrics
+    iterations: int
+    history: list = field(default_factory=list)
+
+
+def barrier_models(jobs, element_size: float) -> list:
+    """The rigid-barrier runs of calibrate (barrier_model with
+    history_steps = 2, src/vehicle.rs:951-973)."""
+    return [Model.barrier(v, element_size=element_size, speed=t.speed, end_time=BARRIER_END).set(history_steps=2)
+            for v, t in jobs]
+
+
+def calibrate_many(jobs: Sequence, element_size: float = TUNED_ELEMENT_SIZE, iterations: int = 8,
+                   solve: Optional[Callable] = None, measure: Optional[Callable] = None,
+                   log: Optional[Callable[[str], None]] = None) -> list:
+    """calibrate (src/vehicle.rs:1006-1042) for several (vehicle, targets)
+    pairs at once: every iteration solves all unfinished vehicles' barrier
+    tests as one batch, then applies the reference's log-space secant step
+    to each (stiffness x r_k^0.7, yield x (1/r_c)^0.7 r_k^0.21, and once
+    KW400 and crush are within 15 %, modulus x clamp((1/r_e)^1.4, 0.5, 2)
+    with the body modulus following); a vehicle stops when KW400 and crush
+    are within 2 % and restitution within 10 %.
+
+    `measure` (list of vehicles and targets -> list of BarrierMetrics)
+    replaces the barrier solves altogether; `solve` (list of models ->
+    results) replaces only the batched GPU solve."""
+    if measure is None:
+        run = solve or solve_batched
+
+        def measure(pairs):
+            models = barrier_models(pairs, element_size)
+            return [barrier_metrics(mdl, r, v.name.decode(), 0, t.window)
+                    for mdl, r, (v, t) in zip(models, run(models), pairs)]
+
+    tuned = [copy_vehicle(v) for v, _ in jobs]
+    last: list = [None] * len(jobs)
+    hist: list = [[] for _ in jobs]
+    active = list(range(len(jobs)))
+    done_at = [0] * len(jobs)
+    for it in range(iterations):
+        if not active:
+            break
+        ms = measure([(tuned[i], jobs[i][1]) for i in active])
+        still = []
+        for i, m in zip(active, ms):
+            t = jobs[i][1]
+            last[i] = m
+            done_at[i] = it + 1
+            hist[i].append((m.kw400, m.max_crush, m.restitution))
+            if log:
+                log(f"calibrate {tuned[i].name.decode()} it {it}: KW {m.kw400 / 1e3:.0f}/{t.kw400 / 1e3:.0f} N/mm, "
+                    f"crush {m.max_crush * 1e3:.0f}/{t.max_crush * 1e3:.0f} mm, e {m.restitution:.3f}/{t.restitution:.3f}")
+            r_c = t.max_crush / m.max_crush
+            r_k = t.kw400 / m.kw400 if m.max_crush >= t.window[1] else 1.0 / max(r_c, 1.0)
+            r_e = t.restitution / max(m.restitution, 1e-3)
+            if abs(r_k - 1.0) < 0.02 and abs(r_c - 1.0) < 0.02 and abs(r_e - 1.0) < 0.1:
+                continue
+            damp = 0.7
+            v = tuned[i]
+            v.curve[1] *= r_k ** damp
+            v.curve[0] *= (1.0 / r_c) ** damp * r_k ** (0.3 * damp)
+            if abs(r_k - 1.0) < 0.15 and abs(r_c - 1.0) < 0.15:
+                v.modulus *= min(max((1.0 / r_e) ** (2.0 * damp), 0.5), 2.0)
+                v.body_modulus = v.modulus
+            still.append(i)
+        active = still
+    return [BarrierFit(tuned[i], last[i], done_at[i], hist[i]) for i in range(len(jobs))]
+
+
+def calibrate(v: _abi.Vehicle, targets: BarrierTargets, element_size: float = TUNED_ELEMENT_SIZE,
+              iterations: int = 8, solve: Optional[Callable] = None, measure: Optional[Callable] = None,
+              log: Optional[Callable[[str], None]] = None) -> BarrierFit:
+    """calibrate (src/vehicle.rs:1006-1042) for one vehicle."""
+    return calibrate_many([(v, targets)], element_size, iterations, solve, measure, log)[0]

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import crushrs_amd as ca
+import oracle_ffi as of
 from crushrs_amd import calibration as cal
 from crushrs_amd import signal as sg
 
@@ -292,3 +293,155 @@ def test_initial_table_reproduces_the_reference_pulse_fit_of_nhtsa_2320():
     for i in (1, 3, 4, 5, 6):
         assert abs(table[i, 1] / want[i, 1] - 1.0) < 5e-3, (i, table[i, 1], want[i, 1])
     assert want[0, 1] > table[0, 1] and want[2, 1] == want[1, 1] < table[2, 1]
+
+
+# ---------------------------------------------------------------- barrier metrics and calibrate
+
+class _Parts:
+    """A stand-in model for barrier_metrics: part names only."""
+
+    def __init__(self, names):
+        self._names = names
+
+    def part_names(self):
+        return list(self._names)
+
+
+class _History:
+    def __init__(self, t, parts):
+        self.history_time = t
+        self.parts = parts
+
+
+def spring_history(k=625e3, m=1000.0, v0=15.0, dt=1e-4, t_end=0.2, split=0.3):
+    """A body moving +x at v0 into a linear spring F = k x from t = 0, as a
+    two-part vehicle history (masses split and 1 - split, same motion)."""
+    w = math.sqrt(k / m)
+    t = np.arange(int(round(t_end / dt)) + 1) * dt
+    tc = math.pi / w
+    x = np.where(t <= tc, v0 / w * np.sin(w * np.minimum(t, tc)), -v0 * (t - tc))
+    v = np.where(t <= tc, v0 * np.cos(w * np.minimum(t, tc)), -v0)
+    P = np.zeros((len(t), 3, 11))
+    for p, mp in ((0, split * m), (1, (1 - split) * m), (2, 50.0)):
+        P[:, p, 0] = mp
+        if p < 2:
+            P[:, p, 1], P[:, p, 4] = x, v
+    return _History(t, P), w
+
+
+def test_barrier_metrics_of_a_linear_spring():
+    """barrier_metrics on a body into a linear spring k: contact where the
+    force first exceeds 2 % of the peak (crush x_c there), loading to the
+    maximum v0/w; KW over (w0, w1) is k (1 + 2 x_c (w1 - w0)/(w1^2 - w0^2))
+    (the curve measured from x_c), peak force k v0/w, mean force
+    k (x_max + x_c)/2 over the loading crush, restitution 1 and delta-v
+    -2 v0.  Only the vehicle's own parts (<name>, <name>_crush) count."""
+    k, m, v0 = 625e3, 1000.0, 15.0
+    h, w = spring_history(k, m, v0)
+    model = _Parts(["car", "car_crush", "wall"])
+    bm = cal.barrier_metrics(model, h, "car")
+    x = h.parts[:, 0, 1]
+    f_exact = k * x
+    xc = x[np.nonzero(f_exact > 0.02 * k * v0 / w)[0][0]]
+    x_max = v0 / w
+    assert abs(bm.max_crush - (x_max - xc)) < 2e-3
+    assert abs(bm.peak_force / (k * x_max) - 1.0) < 1e-3
+    w0, w1 = 0.025, 0.4
+    want_kw = k * (1.0 + 2.0 * xc * (w1 - w0) / (w1 * w1 - w0 * w0))
+    assert abs(bm.kw400 / want_kw - 1.0) < 2e-3, (bm.kw400, want_kw)
+    assert abs(bm.mean_force / (0.5 * k * (x_max + xc)) - 1.0) < 2e-3
+    assert abs(bm.restitution - 1.0) < 1e-9 and abs(bm.delta_v + 2.0 * v0) < 1e-9
+    assert np.all(np.diff(bm.curve[:, 0]) > 0)
+    with pytest.raises(ValueError):
+        cal.barrier_metrics(model, h, "truck")
+
+
+def test_bilinear_curve_targets():
+    """CrushCurve::kw_window and Vehicle::barrier_prediction: KW of
+    F = F_y + k x over (w0, w1) is k + 2 F_y (w1 - w0)/(w1^2 - w0^2); the
+    predicted crush absorbs 1/2 m v^2 exactly."""
+    v = ca.vehicle("neon-raw")
+    fy, kk = v.curve[0], v.curve[1]
+    assert abs(cal.kw_window(v) - (kk + 2 * fy * 0.375 / (0.16 - 0.025 ** 2))) < 1e-6 * kk
+    c, f = cal.barrier_prediction(v, 15.6464)
+    assert abs(fy * c + 0.5 * kk * c * c - 0.5 * v.mass * 15.6464 ** 2) < 1e-6 * v.mass
+    assert abs(f - (fy + kk * c)) < 1e-9 * f
+    t = cal.BarrierTargets.from_vehicle(v, 0.12)
+    assert t.window == (0.025, 0.4) and abs(t.speed - 35 * 0.44704) < 1e-12 and t.max_crush == c
+
+
+def test_calibrate_secant_converges_on_an_analytic_barrier():
+    """calibrate_many with an analytic barrier (the vehicle behaves as its
+    bilinear curve made 30 % stiffer: KW and crush of that curve;
+    restitution falling with the modulus): both vehicles converge to their
+    targets within the reference's tolerances; each iteration measures only
+    the unfinished ones, in one call; the modulus moves only once KW and
+    crush are within 15 %."""
+    def fake(v, t):
+        s = cal.copy_vehicle(v)
+        s.curve[0] *= 1.3
+        s.curve[1] *= 1.3
+        kw = cal.kw_window(s, t.window)
+        crush = cal.barrier_prediction(s, t.speed)[0]
+        e = 0.12 * (v.modulus / 20e6) ** -0.5
+        return cal.BarrierMetrics(kw, crush, 0.0, 0.0, e, 0.0, np.zeros((0, 2)))
+
+    calls = []
+
+    def measure(pairs):
+        calls.append(len(pairs))
+        return [fake(v, t) for v, t in pairs]
+
+    jobs = []
+    for name in ("neon-raw", "silverado-raw"):
+        v = ca.vehicle(name)
+        v.modulus = v.body_modulus = 10e6
+        jobs.append((v, cal.BarrierTargets.from_vehicle(ca.vehicle(name), 0.12)))
+    fits = cal.calibrate_many(jobs, iterations=30, measure=measure)
+    assert calls[0] == 2 and all(c <= 2 for c in calls)
+    for fit, (v0, t) in zip(fits, jobs):
+        m = fit.metrics
+        assert abs(m.kw400 / t.kw400 - 1) < 0.02 and abs(m.max_crush / t.max_crush - 1) < 0.02
+        assert abs(m.restitution / t.restitution - 1) < 0.1
+        assert fit.iterations < 30
+        assert fit.vehicle.body_modulus == fit.vehicle.modulus
+        es = [h[2] for h in fit.history]
+        first_close = next(i for i, (kw, c, e) in enumerate(fit.history)
+                           if abs(t.kw400 / kw - 1) < 0.15 and abs(t.max_crush / c - 1) < 0.15)
+        assert first_close >= 1 and es[first_close] == es[0]  # the curve moves first, the modulus after
+    assert jobs[0][0].modulus == 10e6  # inputs untouched
+
+
+@pytest.mark.parametrize("preset,kw,crush,e", [("silverado", 2563e3, 0.540, 0.119)])
+def test_tuned_silverado_barrier_metrics_match_the_reference(preset, kw, crush, e):
+    """The reference's published metrics of its tuned Silverado at 0.3 m
+    elements, 35 mph rigid barrier (src/vehicle.rs:635-637: KW400 2563
+    N/mm, max crush 540 mm, restitution 0.119, against targets 2550 N/mm
+    and 513 mm), from the reference's default lane path restated (the
+    oracle's AVX2 path) and barrier_metrics.  The targets match exactly;
+    the restated solve gives 2612 N/mm, 538.7 mm and 0.126 (its f64 path
+    2614 N/mm, 538.5 mm, 0.123): 1.9 % above the published KW400, which
+    the bars below allow (the published figures date from the preset's
+    tuning and are not regenerated by the reference's tests)."""
+    m = cal.barrier_models([(ca.vehicle(preset), cal.BarrierTargets(0, 0, 0, 35 * ca.MPH))], ca.TUNED_ELEMENT_SIZE)[0]
+    r = of.run_lanes(m, threads=8)
+    bm = cal.barrier_metrics(m, r, preset)
+    assert abs(bm.kw400 / kw - 1.0) < 0.025, bm.kw400
+    assert abs(bm.max_crush - crush) < 0.005, bm.max_crush
+    assert abs(bm.restitution - e) < 0.01, bm.restitution
+    t = cal.BarrierTargets.from_vehicle(ca.vehicle(preset + "-raw"), 0.12)
+    assert round(t.kw400 / 1e3) == 2550 and round(t.max_crush * 1e3) == 513
+
+
+def test_tuned_neon_reproduces_ncap_barrier_test():
+    """tests/impact.rs:50-60 on the oracle's f64 path: the tuned Neon's
+    simulated KW400 within 5 % and crush within 10 % of the NCAP Neon's
+    curve, restitution between 0.03 and 0.2."""
+    target = ca.vehicle("neon-raw")
+    t = cal.BarrierTargets.from_vehicle(target, 0.12)
+    m = cal.barrier_models([(ca.vehicle("neon"), t)], ca.TUNED_ELEMENT_SIZE)[0]
+    r = of.run(m)
+    bm = cal.barrier_metrics(m, r, "neon")
+    assert abs(bm.kw400 - t.kw400) < 0.05 * t.kw400, (bm.kw400, t.kw400)
+    assert abs(bm.max_crush - t.max_crush) < 0.10 * t.max_crush, (bm.max_crush, t.max_crush)
+    assert 0.03 < bm.restitution < 0.2
