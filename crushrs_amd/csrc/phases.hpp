@@ -29,6 +29,18 @@ __device__ __forceinline__ double ld_f64(rsrc_t r, unsigned voff, unsigned soff)
 __device__ __forceinline__ void st_f32(rsrc_t r, float v, unsigned voff, unsigned soff) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
 }
+typedef unsigned u32x3 __attribute__((ext_vector_type(3)));
+struct Float3 {
+    float x, y, z;
+};
+// Three consecutive floats in one 12-byte store.
+__device__ __forceinline__ void st_f32x3(rsrc_t r, float a, float b, float c, unsigned voff, unsigned soff) {
+    u32x3 v;
+    v.x = __builtin_bit_cast(unsigned, a);
+    v.y = __builtin_bit_cast(unsigned, b);
+    v.z = __builtin_bit_cast(unsigned, c);
+    __builtin_amdgcn_raw_buffer_store_b96(v, r, voff, soff, 0);
+}
 
 // Everything a step touches for one model (one instance), as device
 // pointers.  The one-solve path fills it from its buffers; the batched path
@@ -340,7 +352,8 @@ __device__ __forceinline__ void contact_apply_node(const InstDev& D, int i) {
 // Returns false when the new displacement is not finite.
 __device__ __forceinline__ bool half_node(const InstDev& D, int n, double dt) {
 #pragma clang fp contract(off)
-    if (D.fixed[n]) {
+    const uint8_t fl = D.fixed[n];
+    if (fl & NODE_FIXED) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             D.u[3 * (size_t)n + d] = 0.0;
@@ -353,7 +366,8 @@ __device__ __forceinline__ bool half_node(const InstDev& D, int n, double dt) {
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const size_t k = 3 * (size_t)n + d;
-        const double a = (D.f_ext[k] - D.f_int[k]) * im;
+        const double fx = (fl & NODE_CONTACT) ? D.f_ext[k] : 0.0;
+        const double a = (fx - D.f_int[k]) * im;
         const double vh = D.v[k] + 0.5 * dt * a;
         D.v[k] = vh;
         const double un = D.u[k] + dt * vh;
@@ -372,11 +386,13 @@ __device__ __forceinline__ bool half_node(const InstDev& D, int n, double dt) {
 template <int KIND>
 __device__ __forceinline__ int force_element(const InstDev& D, int s) {
     const unsigned E = (unsigned)D.E, C = (unsigned)D.C, nh = (unsigned)D.n_hc;
+    // corner forces: fe[(corner * E + slot) * 3 + axis], each corner's xyz
+    // in one 12-byte store (and one load in the gather)
     const rsrc_t rfe = make_rsrc(D.fe, (size_t)24 * E * 4);
-    const unsigned vs = (unsigned)s * 4u;
+    const unsigned vs = (unsigned)s * 4u, vs3 = (unsigned)s * 12u;
     if (D.eroded[s]) {
 #pragma unroll
-        for (int k = 0; k < 24; ++k) st_f32(rfe, 0.0f, vs, k * E * 4u);
+        for (int i = 0; i < 8; ++i) st_f32x3(rfe, 0.0f, 0.0f, 0.0f, vs3, i * E * 12u);
         return 0;
     }
     const rsrc_t rconn = make_rsrc(D.conn, (size_t)8 * E * 4);
@@ -429,7 +445,7 @@ __device__ __forceinline__ int force_element(const InstDev& D, int s) {
     if (!(J > 0.0f)) {
         D.eroded[s] = 1;
 #pragma unroll
-        for (int k = 0; k < 24; ++k) st_f32(rfe, 0.0f, vs, k * E * 4u);
+        for (int i = 0; i < 8; ++i) st_f32x3(rfe, 0.0f, 0.0f, 0.0f, vs3, i * E * 12u);
         return 1;
     }
     // r = H q with H packed as its upper triangle
@@ -473,13 +489,14 @@ __device__ __forceinline__ int force_element(const InstDev& D, int s) {
         float ga[4];
 #pragma unroll
         for (int a = 0; a < 4; ++a) ga[a] = ld_f32(rgam, vc, (4 * i + a) * C * 4u);
+        float f[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            float f = M[3 * d] * g0 + M[3 * d + 1] * g1 + M[3 * d + 2] * g2;
+            f[d] = M[3 * d] * g0 + M[3 * d + 1] * g1 + M[3 * d + 2] * g2;
 #pragma unroll
-            for (int a = 0; a < 4; ++a) f += rr[3 * a + d] * ga[a];
-            st_f32(rfe, f, vs, (3 * i + d) * E * 4u);
+            for (int a = 0; a < 4; ++a) f[d] += rr[3 * a + d] * ga[a];
         }
+        st_f32x3(rfe, f[0], f[1], f[2], vs3, i * E * 12u);
     }
     return 0;
 }
@@ -502,9 +519,14 @@ __device__ __forceinline__ bool gather_node(const InstDev& D, int n, double dt, 
     for (int j = 0; j < 8; ++j) code[j] = kb + j < ke ? D.n2e_idx[kb + j] : -1;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-        const size_t base = code[j] >= 0 ? (size_t)(3 * (code[j] & 7)) * E + (code[j] >> 3) : 0;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) g[j][d] = code[j] >= 0 ? D.fe[base + (size_t)d * E] : 0.0f;
+        // one 12-byte load per incidence (element 0 for a missing one).  A
+        // plain struct load: this toolchain's raw_buffer_load_b96 builtin
+        // keeps only the first dword when the elements are extracted.
+        const size_t base = code[j] >= 0 ? ((size_t)(code[j] & 7) * E + (code[j] >> 3)) * 3 : 0;
+        const Float3 w = *reinterpret_cast<const Float3*>(D.fe + base);
+        g[j][0] = code[j] >= 0 ? w.x : 0.0f;
+        g[j][1] = code[j] >= 0 ? w.y : 0.0f;
+        g[j][2] = code[j] >= 0 ? w.z : 0.0f;
     }
 #pragma unroll
     for (int j = 0; j < 8; ++j)
@@ -513,18 +535,21 @@ __device__ __forceinline__ bool gather_node(const InstDev& D, int n, double dt, 
             for (int d = 0; d < 3; ++d) f[d] += (double)g[j][d];
     for (int k = kb + 8; k < ke; ++k) {
         const int cd = D.n2e_idx[k];
-        const size_t base = (size_t)(3 * (cd & 7)) * E + (cd >> 3);
-#pragma unroll
-        for (int d = 0; d < 3; ++d) f[d] += (double)D.fe[base + (size_t)d * E];
+        const Float3 w = *reinterpret_cast<const Float3*>(D.fe + ((size_t)(cd & 7) * E + (cd >> 3)) * 3);
+        f[0] += (double)w.x;
+        f[1] += (double)w.y;
+        f[2] += (double)w.z;
     }
     const double im = D.inv_mass[n];
-    const bool fx = D.fixed[n] != 0;
+    const uint8_t fl = D.fixed[n];
+    const bool fx = (fl & NODE_FIXED) != 0;
     bool ok = true;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         const size_t k = 3 * (size_t)n + d;
         D.f_int[k] = f[d];
-        const double a = (D.f_ext[k] - f[d]) * im;
+        const double fe = (fl & NODE_CONTACT) ? D.f_ext[k] : 0.0;
+        const double a = (fe - f[d]) * im;
         double vn = (D.v[k] + 0.5 * dt * a) * damp;
         if (fx) vn = 0.0;
         D.v[k] = vn;
@@ -731,7 +756,7 @@ __device__ void group_means(const InstDev& D, int g, double* s_w, double* out) {
 __device__ __forceinline__ double accel_local_x(const InstDev& D, int g) {
 #pragma clang fp contract(off)
     const int n = D.g_accel[g];
-    if (n < 0 || D.fixed[n]) return 0.0;
+    if (n < 0 || (D.fixed[n] & NODE_FIXED)) return 0.0;
     const double im = D.inv_mass[n];
     double a[3];
 #pragma unroll
@@ -841,7 +866,7 @@ __device__ __forceinline__ void node_position(const InstDev& D, int n, double* x
 // (src/solver.rs:333-342).
 __device__ __forceinline__ void node_accel(const InstDev& D, int n, double* a) {
 #pragma clang fp contract(off)
-    const bool fx = D.fixed[n] != 0;
+    const bool fx = (D.fixed[n] & NODE_FIXED) != 0;
     const double im = D.inv_mass[n];
 #pragma unroll
     for (int d = 0; d < 3; ++d) a[d] = fx ? 0.0 : (D.f_ext[3 * (size_t)n + d] - D.f_int[3 * (size_t)n + d]) * im;

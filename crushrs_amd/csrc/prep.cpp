@@ -426,7 +426,7 @@ void prepare(const crush_model& m, Prepared& p) {
     for (int i = 0; i < m.n_fixed; ++i) {
         int n = m.fixed_nodes[i];
         if (n < 0 || n >= N) fail(CRUSH_E_INPUT, "fixed node %d out of range", n);
-        p.fixed[n] = 1;
+        p.fixed[n] |= NODE_FIXED;
     }
     // node -> element incidence in original element order
     p.n2e_ptr.assign(N + 1, 0);
@@ -509,6 +509,7 @@ void prepare(const crush_model& m, Prepared& p) {
         for (int n = 0; n < N; ++n) {
             if (lists[n].empty()) continue;
             p.tn_node.push_back(n);
+            p.fixed[n] |= NODE_CONTACT;
             p.tn_ent.insert(p.tn_ent.end(), lists[n].begin(), lists[n].end());
             p.tn_ptr.push_back((int32_t)p.tn_ent.size());
         }

@@ -14,6 +14,9 @@ namespace crush {
 constexpr int HSYM = 78;        // packed upper triangle of the 12x12 H
 constexpr int SEARCH_BLOCK = 256;
 constexpr int FACE_HITS_CAP = 16;  // recorded hits per face before the scan fallback
+// Node flags (Prepared::fixed): a fixed node; a node contact forces reach
+// (its f_ext is written every step; every other node's stays 0).
+constexpr uint8_t NODE_FIXED = 1, NODE_CONTACT = 2;
 
 // Per-material parameters as the force kernels read them (f32).
 struct MatParams {
@@ -54,7 +57,7 @@ struct Prepared {
     std::vector<double> elem_mass;    // [E] by slot: V rho
     std::vector<MatParams> mats;      // [parts]
     std::vector<double> X, v0, mass;  // [3N], [3N], [N]
-    std::vector<uint8_t> fixed;       // [N]
+    std::vector<uint8_t> fixed;       // [N] NODE_FIXED | NODE_CONTACT
     // node -> (slot << 3 | corner), incidences in original element order
     std::vector<int32_t> n2e_ptr, n2e_idx;
     // contacts
