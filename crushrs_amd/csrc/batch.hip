@@ -145,6 +145,7 @@ ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     o.cn_scale = b.put(P.cn_scale);
     o.cf_nodes = b.put(P.cf_nodes);
     o.cf_contact = b.put(P.cf_contact);
+    o.cf_static = b.put(P.cf_static);
     o.tn_node = b.put(P.tn_node);
     o.tn_ptr = b.put(P.tn_ptr);
     o.tn_ent = b.put(P.tn_ent);
@@ -236,6 +237,7 @@ InstDev make_inst(const Prepared& P, const ImageOffsets& o, uintptr_t base) {
     I.cn_scale = at<double>(base, o.cn_scale);
     I.cf_nodes = at<int32_t>(base, o.cf_nodes);
     I.cf_contact = at<int32_t>(base, o.cf_contact);
+    I.cf_static = at<uint8_t>(base, o.cf_static);
     I.tn_node = at<int32_t>(base, o.tn_node);
     I.tn_ptr = at<int32_t>(base, o.tn_ptr);
     I.tn_ent = at<int32_t>(base, o.tn_ent);
@@ -406,14 +408,20 @@ __device__ __forceinline__ void grid_contact(const InstDev* __restrict__ table, 
     const bool can_grow = cd.n_faces > BATCH_GRID_LARGE;
     const int mode = can_grow ? D.cgp[ci].mode : 0;
     if (BIG ? mode == 0 : mode != 0) return;  // the other kernel builds this grid
+    // the grid of an earlier step serves while every face stays inside its
+    // box at that build grown by the margin; faces of fixed nodes never move
+    const FaceGrid prev = D.cgp[ci];
+    const bool reusable = prev.built && !prev.overflow;
+    if (reusable && cd.all_static) return;
     for (int f = threadIdx.x; f < cd.n_faces; f += B) contact_face(D, cd.face_begin + f);
     __syncthreads();
+    if (reusable && grid_still_valid<B>(D, cd, prev.margin)) return;
     int32_t* gs = D.cgrid + cd.grid_off;
     int32_t* ge = gs + cd.grid_cells + 1;
     FaceGrid g;
     int next = mode;
     if (mode < 2) {
-        g = build_face_grid<B>(D, cd, cell_scale, MAXC, MAXE, s_start, s_cursor, s_entries, s_w, s_tmp);
+        g = build_face_grid<B>(D, cd, cell_scale, MAXC, MAXE, s_start, s_cursor, s_entries, s_w, s_tmp, GRID_MARGIN);
         if (!g.overflow) {
             const int ncell = g.dim[0] * g.dim[1] * g.dim[2];
             for (int i = threadIdx.x; i <= ncell; i += B) gs[i] = s_start[i];
@@ -426,7 +434,7 @@ __device__ __forceinline__ void grid_contact(const InstDev* __restrict__ table, 
     if (BIG && next == 2) {
         __syncthreads();
         g = build_face_grid<B>(D, cd, cell_scale, cd.grid_cells, cd.grid_entries, gs, ge + cd.grid_entries, ge, s_w,
-                               s_tmp);
+                               s_tmp, GRID_MARGIN);
     }
     g.mode = next;
     if (threadIdx.x == 0) D.cgp[ci] = g;

@@ -54,8 +54,12 @@ struct StepCtl {
 // Face geometry record width (doubles) in the SoA face array:
 // x0..x3 (12), normal (3), centre (3), inflated lo (3), hi (3),
 // candidate lo (3), candidate hi (3), size (1).
-constexpr int FG = 31;
+constexpr int FG = 37;
 constexpr int FG_N = 12, FG_C = 15, FG_LO = 18, FG_HI = 21, FG_CLO = 24, FG_CHI = 27, FG_SZ = 30;
+// batched path: the candidate box a face had when its contact's grid was
+// last built (a reused grid stays valid while every box stays within it
+// plus the grid's margin)
+constexpr int FG_BLO = 31, FG_BHI = 34;
 
 // Contact constants: src/contact.rs:75 (MAX_ACCEL), :87 (SOFT), :34 (edge
 // tolerance of the triangle test).
@@ -519,7 +523,7 @@ CRUSH_HD bool face_accepts(const double* __restrict__ fg, int F, int f, const do
 // map, so the corners suffice); the node itself lies up to max_distance
 // behind that.  Ill-conditioned faces fall back to the inflated AABB, which
 // the acceptance test applies first anyway.
-CRUSH_HD void face_geometry(const double (*x)[3], double max_d, double* out /* [FG] */) {
+CRUSH_HD void face_geometry(const double (*x)[3], double max_d, double* out /* [FG_BLO] */) {
 #pragma clang fp contract(off)
     double e1[3], e2[3], n[3];
 #pragma unroll

@@ -82,6 +82,7 @@ struct InstDev {
     const double* cn_scale;
     const int32_t* cf_nodes;
     const int32_t* cf_contact;
+    const uint8_t* cf_static;
     const int32_t* tn_node;
     const int32_t* tn_ptr;
     const int32_t* tn_ent;
@@ -166,6 +167,8 @@ __device__ __forceinline__ void mass_node(const InstDev& D, int n) {
 // Geometry of contact face f for the current configuration.
 __device__ __forceinline__ void contact_face(const InstDev& D, int f) {
 #pragma clang fp contract(off)
+    // a face of fixed nodes keeps the geometry of the first step
+    if (D.cf_static[f] && D.ctl->step > 0) return;
     double x[4][3];
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -173,11 +176,10 @@ __device__ __forceinline__ void contact_face(const InstDev& D, int f) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) x[k][d] = D.X[3 * (size_t)n + d] + D.u[3 * (size_t)n + d];
     }
-    double g[FG];
+    double g[FG_BLO];
     face_geometry(x, D.cdesc[D.cf_contact[f]].max_distance, g);
 #pragma unroll
-    for (int k = 0; k < FG; ++k) D.fgeo[(size_t)k * D.F + f] = g[k];
-    D.fh_count[f] = 0;
+    for (int k = 0; k < FG_BLO; ++k) D.fgeo[(size_t)k * D.F + f] = g[k];
 }
 
 // Grid over one contact's face candidate boxes (built in LDS).
@@ -190,7 +192,8 @@ struct FaceGrid {
     // grid is built, 0 small LDS grid, 1 large LDS grid, 2 device memory;
     // raised when a grid overflows and kept at later steps
     int mode;
-    int reserved;
+    int built;      // the grid arrays hold a grid of these parameters
+    double margin;  // face boxes were binned grown by this much (batched path)
 };
 
 __device__ __forceinline__ int grid_cell(const FaceGrid& g, double x, int d) {
@@ -310,11 +313,14 @@ __device__ __forceinline__ void face_reaction(const InstDev& D, int f, double* s
     }
 }
 
+// Reaction of face f, and its hit count cleared for the next step's search
+// (the counts start at zero).
 __device__ __forceinline__ void contact_face_force(const InstDev& D, int f) {
     double s[3];
     face_reaction(D, f, s);
 #pragma unroll
     for (int d = 0; d < 3; ++d) D.face_force[3 * (size_t)f + d] = s[d];
+    D.fh_count[f] = 0;
 }
 
 // External (contact) force of touched node i, in contact order.  With
@@ -622,10 +628,16 @@ __device__ void block_scan(int* s, int n, int* s_tmp) {
 // within a cell is irrelevant: the search keeps the lowest index).  The
 // arrays are in LDS, or in device memory for grids larger than LDS holds;
 // MAXC and MAXE are their capacities.
+//
+// margin_frac > 0 (the batched path, which keeps a grid over steps): every
+// face is binned with its candidate box grown by margin = margin_frac x the
+// largest face extent, and that box is recorded (FG_BLO, FG_BHI); the grid
+// serves later steps as long as every face's box stays inside its recorded
+// box grown by the margin (grid_still_valid).
 template <int B>
 __device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, double cell_scale, const int MAXC,
                                     const int MAXE, int* s_start, int* s_cursor, int* s_entries, double* s_w,
-                                    int* s_tmp) {
+                                    int* s_tmp, double margin_frac = 0.0) {
     const int t = threadIdx.x, F = D.F, fb = cd.face_begin, nf = cd.n_faces;
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY}, sz = 0.0;
     for (int f = t; f < nf; f += B) {
@@ -644,6 +656,12 @@ __device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, dou
         ghi[d] = block_reduce<B>(hi[d], 2, s_w);
     }
     sz = block_reduce<B>(sz, 2, s_w);
+    const double m = margin_frac * sz;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        g.glo[d] -= m;
+        ghi[d] += m;
+    }
     double h = fmax(sz * cell_scale, 1e-9);
     // cell size: grown until the grid has at most MAXC cells; when the
     // faces' candidate boxes then make more than MAXE entries, doubled and
@@ -664,7 +682,8 @@ __device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, dou
         }
         g.h = h;
         g.mode = 0;
-        g.reserved = 0;
+        g.built = 1;
+        g.margin = m;
         g.overflow = (nf == 0 || cells > MAXC || !(isfinite(g.glo[0]) && isfinite(ghi[0]))) ? 1 : 0;
         const int dx = g.dim[0], dy = g.dim[1];
         ncell = g.overflow ? 1 : (int)cells;
@@ -680,9 +699,9 @@ __device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, dou
                     fhi[d] = D.fgeo[(size_t)(FG_CHI + d) * F + fb + f];
                 }
                 if (!(flo[0] <= fhi[0] && flo[1] <= fhi[1] && flo[2] <= fhi[2])) continue;
-                const int a0 = grid_cell(g, flo[0], 0), a1 = grid_cell(g, fhi[0], 0);
-                const int b0 = grid_cell(g, flo[1], 1), b1 = grid_cell(g, fhi[1], 1);
-                const int c0 = grid_cell(g, flo[2], 2), c1 = grid_cell(g, fhi[2], 2);
+                const int a0 = grid_cell(g, flo[0] - m, 0), a1 = grid_cell(g, fhi[0] + m, 0);
+                const int b0 = grid_cell(g, flo[1] - m, 1), b1 = grid_cell(g, fhi[1] + m, 1);
+                const int c0 = grid_cell(g, flo[2] - m, 2), c1 = grid_cell(g, fhi[2] + m, 2);
                 for (int kz = c0; kz <= c1; ++kz)
                     for (int ky = b0; ky <= b1; ++ky)
                         for (int kx = a0; kx <= a1; ++kx) atomicAdd(&s_start[(kz * dy + ky) * dx + kx], 1);
@@ -705,10 +724,16 @@ __device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, dou
                 flo[d] = D.fgeo[(size_t)(FG_CLO + d) * F + fb + f];
                 fhi[d] = D.fgeo[(size_t)(FG_CHI + d) * F + fb + f];
             }
+            if (margin_frac > 0.0)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    D.fgeo[(size_t)(FG_BLO + d) * F + fb + f] = flo[d];
+                    D.fgeo[(size_t)(FG_BHI + d) * F + fb + f] = fhi[d];
+                }
             if (!(flo[0] <= fhi[0] && flo[1] <= fhi[1] && flo[2] <= fhi[2])) continue;
-            const int a0 = grid_cell(g, flo[0], 0), a1 = grid_cell(g, fhi[0], 0);
-            const int b0 = grid_cell(g, flo[1], 1), b1 = grid_cell(g, fhi[1], 1);
-            const int c0 = grid_cell(g, flo[2], 2), c1 = grid_cell(g, fhi[2], 2);
+            const int a0 = grid_cell(g, flo[0] - m, 0), a1 = grid_cell(g, fhi[0] + m, 0);
+            const int b0 = grid_cell(g, flo[1] - m, 1), b1 = grid_cell(g, fhi[1] + m, 1);
+            const int c0 = grid_cell(g, flo[2] - m, 2), c1 = grid_cell(g, fhi[2] + m, 2);
             for (int kz = c0; kz <= c1; ++kz)
                 for (int ky = b0; ky <= b1; ++ky)
                     for (int kx = a0; kx <= a1; ++kx) s_entries[atomicAdd(&s_cursor[(kz * dy + ky) * dx + kx], 1)] = f;
@@ -716,6 +741,29 @@ __device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, dou
     }
     __syncthreads();
     return g;
+}
+
+// Whether the grid built with margin `m` still serves this step: every
+// face whose candidate box is valid has it inside the box recorded at the
+// build grown by m (then every cell a node in that box maps to lists the
+// face).  Every thread of the block calls it.
+template <int B>
+__device__ bool grid_still_valid(const InstDev& D, const ContactDesc& cd, double m) {
+    const int F = D.F, fb = cd.face_begin;
+    int stale = 0;
+    for (int f = threadIdx.x; f < cd.n_faces; f += B) {
+        if (D.cf_static[fb + f]) continue;
+        bool valid = true, inside = true;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const double lo = D.fgeo[(size_t)(FG_CLO + d) * F + fb + f], hi = D.fgeo[(size_t)(FG_CHI + d) * F + fb + f];
+            valid = valid && lo <= hi;
+            inside = inside && lo >= D.fgeo[(size_t)(FG_BLO + d) * F + fb + f] - m &&
+                     hi <= D.fgeo[(size_t)(FG_BHI + d) * F + fb + f] + m;
+        }
+        if (valid && !inside) stale = 1;
+    }
+    return !__syncthreads_or(stale);
 }
 
 // Crush of group g, |u_axis(accel) - mean u_axis(front)| (src/bin/crushrs.rs:265-271).

@@ -494,6 +494,17 @@ void prepare(const crush_model& m, Prepared& p) {
             for (int k = 0; k < 4; ++k)
                 p.cf_nodes[(size_t)k * p.n_cfaces + p.cdesc[c].face_begin + f] = m.faces[4 * ct.faces[f] + k];
     }
+    // faces of fixed nodes only (rigid walls and roads) keep their geometry
+    p.cf_static.assign((size_t)p.n_cfaces, 0);
+    for (int f = 0; f < p.n_cfaces; ++f) {
+        bool st = true;
+        for (int k = 0; k < 4; ++k) st = st && (p.fixed[p.cf_nodes[(size_t)k * p.n_cfaces + f]] & NODE_FIXED);
+        p.cf_static[f] = st ? 1 : 0;
+    }
+    for (ContactDesc& d : p.cdesc) {
+        d.all_static = 1;
+        for (int f = 0; f < d.n_faces; ++f) d.all_static &= p.cf_static[d.face_begin + f];
+    }
     // apply lists: per touched node, in contact order: own record, then
     // the faces containing it
     {

@@ -39,6 +39,7 @@ struct ContactDesc {
     // array (offset in ints) and its capacities in cells and entries
     int32_t grid_cells;
     int32_t grid_entries, grid_off;
+    int32_t all_static;  // every face of the contact is of fixed nodes
 };
 
 struct Prepared {
@@ -67,6 +68,7 @@ struct Prepared {
     std::vector<double> cn_scale;     // [K]
     std::vector<int32_t> cf_nodes;    // [4][F] node ids of each contact face (SoA)
     std::vector<int32_t> cf_contact;  // [F]
+    std::vector<uint8_t> cf_static;   // [F] 1: every node of the face is fixed
     std::vector<int32_t> tn_node, tn_ptr, tn_ent;  // contact apply lists
     std::vector<int32_t> sblk_contact, sblk_start;  // search blocks
     int n_cnodes = 0, n_cfaces = 0;

@@ -29,7 +29,7 @@ struct DeviceModel {
     DevBuf<int32_t> conn, ecls, cls_mat, n2e_ptr, n2e_idx;
     DevBuf<float> cls_grad, cls_vol, cls_gamma, cls_H, state, fe;
     DevBuf<double> cls_dtf, elem_mass, extra, X, u, v, f_int, f_ext, inv_mass, mass, added, part_min, part_sum;
-    DevBuf<uint8_t> eroded, fixed;
+    DevBuf<uint8_t> eroded, fixed, cf_static;
     DevBuf<MatParams> mats;
     DevBuf<ContactDesc> cdesc;
     DevBuf<int32_t> cn_node, cn_contact, cf_nodes, cf_contact, tn_node, tn_ptr, tn_ent, sblk_contact, sblk_start;
@@ -88,6 +88,7 @@ struct DeviceModel {
         cn_scale.upload(P.cn_scale, s);
         cf_nodes.upload(P.cf_nodes, s);
         cf_contact.upload(P.cf_contact, s);
+        cf_static.upload(P.cf_static, s);
         tn_node.upload(P.tn_node, s);
         tn_ptr.upload(P.tn_ptr, s);
         tn_ent.upload(P.tn_ent, s);
@@ -171,6 +172,7 @@ struct DeviceModel {
         I.cn_scale = cn_scale.p;
         I.cf_nodes = cf_nodes.p;
         I.cf_contact = cf_contact.p;
+        I.cf_static = cf_static.p;
         I.tn_node = tn_node.p;
         I.tn_ptr = tn_ptr.p;
         I.tn_ent = tn_ent.p;
