@@ -644,7 +644,7 @@ def main():
     ap.add_argument("--workload", default="sweep", choices=["sweep", "calib", "large", "crash", "serve", "pulse-fit"])
     ap.add_argument("--rates", default="10,50,200,800,2000", help="serve: offered loads, requests/s")
     ap.add_argument("--requests", type=int, default=400, help="serve: requests per offered load")
-    ap.add_argument("--top-requests", type=int, default=2000, help="serve: requests at the highest offered load")
+    ap.add_argument("--top-requests", type=int, default=4096, help="serve: requests at the highest offered load")
     ap.add_argument("--pool", type=int, default=256, help="serve: distinct request models")
     ap.add_argument("--hbm-budget-gib", type=float, default=32.0, help="serve: device memory for resident requests")
     ap.add_argument("--max-resident", type=int, default=4096, help="serve: requests on the device at once")

@@ -360,17 +360,23 @@ __global__ __launch_bounds__(B) void k_end(const InstDev* __restrict__ table, co
             }
         }
     }
+    // stable step and selective mass scaling (src/solver.rs:288-299) from
+    // the force tiles' partials, reduced by the whole block
+    const bool check = !c->done && c->adaptive && c->step % c->chk == 0;
+    double mn = INFINITY, sm = 0.0;
+    if (check) {
+        const int T = force_tile;
+        const int slots = (D.n_hc + T - 1) / T + (D.E - D.n_hc + T - 1) / T;
+        for (int j = t; j < slots; j += B) {
+            mn = fmin(mn, D.dtp_min[j]);
+            sm += D.dtp_sum[j];
+        }
+        mn = block_reduce<B>(mn, 1, s_w);
+        sm = block_reduce<B>(sm, 0, s_w);
+    }
     if (t == 0 && !c->done) {
         c->scale_now = 0;
-        if (c->adaptive && c->step % c->chk == 0) {
-            // stable step and selective mass scaling (src/solver.rs:288-299)
-            const int T = force_tile;
-            const int slots = (D.n_hc + T - 1) / T + (D.E - D.n_hc + T - 1) / T;
-            double mn = INFINITY, sm = 0.0;
-            for (int j = 0; j < slots; ++j) {
-                mn = fmin(mn, D.dtp_min[j]);
-                sm += D.dtp_sum[j];
-            }
+        if (check) {
             if (isfinite(mn)) {
                 const bool scale = c->dt_floor > 0.0 && mn < c->dt_floor;
                 if (scale) {
