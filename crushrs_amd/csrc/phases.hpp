@@ -454,21 +454,6 @@ __device__ __forceinline__ int force_element(const InstDev& D, int s) {
         for (int i = 0; i < 8; ++i) st_f32x3(rfe, 0.0f, 0.0f, 0.0f, vs3, i * E * 12u);
         return 1;
     }
-    // r = H q with H packed as its upper triangle
-    float rr[12];
-#pragma unroll
-    for (int k = 0; k < 12; ++k) rr[k] = 0.0f;
-    {
-        int kk = 0;
-#pragma unroll
-        for (int r = 0; r < 12; ++r)
-#pragma unroll
-            for (int cc = r; cc < 12; ++cc) {
-                const float h = ld_f32(rhs, vc, (unsigned)(kk++) * C * 4u);
-                rr[r] += h * q[cc];
-                if (cc != r) rr[cc] += h * q[r];
-            }
-    }
     const MatParams& mp = D.mats[D.cls_mat[c]];
     float tau[9];
     if (KIND == CRUSH_MAT_HONEYCOMB) {
@@ -487,6 +472,21 @@ __device__ __forceinline__ int force_element(const InstDev& D, int s) {
 #pragma unroll
         for (int j = 0; j < 3; ++j)
             M[3 * i + j] = vol * (tau[3 * i] * finv[3 * j] + tau[3 * i + 1] * finv[3 * j + 1] + tau[3 * i + 2] * finv[3 * j + 2]);
+    // r = H q with H packed as its upper triangle
+    float rr[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) rr[k] = 0.0f;
+    {
+        int kk = 0;
+#pragma unroll
+        for (int r = 0; r < 12; ++r)
+#pragma unroll
+            for (int cc = r; cc < 12; ++cc) {
+                const float h = ld_f32(rhs, vc, (unsigned)(kk++) * C * 4u);
+                rr[r] += h * q[cc];
+                if (cc != r) rr[cc] += h * q[r];
+            }
+    }
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         const float g0 = ld_f32(rgrad, vc, (3 * i) * C * 4u);
