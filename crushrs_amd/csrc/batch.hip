@@ -130,15 +130,10 @@ ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     std::vector<ContactDesc> cdesc = P.cdesc;
     size_t grid_ints = 0;
     for (ContactDesc& cd : cdesc) {
-        cd.grid_cells = BATCH_GRID_CELLS;
-        cd.grid_entries = BATCH_GRID_ENTRIES;
-        const bool big = cd.n_faces > BATCH_GRID_LARGE;
-        if (big) {
-            cd.grid_cells = std::min(std::max(8 * cd.n_faces, BIG_GRID_CELLS), 1 << 16);
-            cd.grid_entries = std::min(std::max(64 * cd.n_faces, BIG_GRID_ENTRIES), 1 << 21);
-        }
+        cd.grid_cells = std::min(std::max(8 * cd.n_faces, BATCH_GRID_CELLS), 1 << 16);
+        cd.grid_entries = std::min(std::max(64 * cd.n_faces, BATCH_GRID_ENTRIES), 1 << 21);
         cd.grid_off = (int32_t)grid_ints;
-        grid_ints += (size_t)cd.grid_cells + 1 + cd.grid_entries + (big ? cd.grid_cells : 0);
+        grid_ints += (size_t)cd.grid_cells + 1 + cd.grid_entries + cd.grid_cells;
     }
     o.cdesc = b.put(cdesc);
     o.cn_node = b.put(P.cn_node);
@@ -393,29 +388,25 @@ __global__ __launch_bounds__(B) void k_end(const InstDev* __restrict__ table, co
 }
 
 // Contact (src/contact.rs:92-160), in four launches over all open steps.
-// (1) geometry of the faces of one contact of one instance and its face
-// grid, built in LDS and stored to device memory: a 1024-cell grid
-// (k_grid, every contact); for a contact of more than BATCH_GRID_LARGE
-// faces whose grid overflowed it, a 2048-cell grid in 64 KB of LDS
-// (k_grid_big, launched after k_grid in the same step); when that overflows
-// too, a grid built in the contact's device-memory region.  A contact stays
-// at the level it reached.  The grid layout never changes which face a node
-// hits: the search keeps the lowest accepting face index.
-template <int B, int MAXC, int MAXE, bool BIG>
-__device__ __forceinline__ void grid_contact(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
-                                             const int32_t* __restrict__ n_tiles, double cell_scale, int* s_start,
-                                             int* s_cursor, int* s_entries, double* s_w, int* s_tmp) {
+// (1) geometry of the faces of one contact of one instance, and its face
+// grid in the contact's device-memory region (8 cells and 64 entries per
+// face, at least BATCH_GRID_CELLS and BATCH_GRID_ENTRIES).  A grid is kept
+// over steps while every face stays inside its box at the build grown by
+// the grid's margin, so builds are rare and need no LDS.  The grid layout
+// never changes which face a node hits: the search keeps the lowest
+// accepting face index.
+template <int B>
+__global__ __launch_bounds__(B) void k_grid(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
+                                            const int32_t* __restrict__ n_tiles, double cell_scale) {
+    __shared__ double s_w[B / 64];
+    __shared__ int s_tmp[B / 64];
     if ((int)blockIdx.x >= *n_tiles) return;
     const int2 tile = tiles[blockIdx.x];
     const InstDev& D = table[tile.x];
     if (!D.ctl->pending) return;
     const int ci = tile.y;
     const ContactDesc cd = D.cdesc[ci];
-    const bool can_grow = cd.n_faces > BATCH_GRID_LARGE;
-    const int mode = can_grow ? D.cgp[ci].mode : 0;
-    if (BIG ? mode == 0 : mode != 0) return;  // the other kernel builds this grid
-    // the grid of an earlier step serves while every face stays inside its
-    // box at that build grown by the margin; faces of fixed nodes never move
+    // faces of fixed nodes never move
     const FaceGrid prev = D.cgp[ci];
     const bool reusable = prev.built && !prev.overflow;
     if (reusable && cd.all_static) return;
@@ -424,50 +415,9 @@ __device__ __forceinline__ void grid_contact(const InstDev* __restrict__ table, 
     if (reusable && grid_still_valid<B>(D, cd, prev.margin)) return;
     int32_t* gs = D.cgrid + cd.grid_off;
     int32_t* ge = gs + cd.grid_cells + 1;
-    FaceGrid g;
-    int next = mode;
-    if (mode < 2) {
-        g = build_face_grid<B>(D, cd, cell_scale, MAXC, MAXE, s_start, s_cursor, s_entries, s_w, s_tmp, GRID_MARGIN);
-        if (!g.overflow) {
-            const int ncell = g.dim[0] * g.dim[1] * g.dim[2];
-            for (int i = threadIdx.x; i <= ncell; i += B) gs[i] = s_start[i];
-            const int total = s_start[ncell];
-            for (int i = threadIdx.x; i < total; i += B) ge[i] = s_entries[i];
-        } else if (can_grow) {
-            next = mode + 1;
-        }
-    }
-    if (BIG && next == 2) {
-        __syncthreads();
-        g = build_face_grid<B>(D, cd, cell_scale, cd.grid_cells, cd.grid_entries, gs, ge + cd.grid_entries, ge, s_w,
-                               s_tmp, GRID_MARGIN);
-    }
-    g.mode = next;
+    const FaceGrid g = build_face_grid<B>(D, cd, cell_scale, cd.grid_cells, cd.grid_entries, gs,
+                                          ge + cd.grid_entries, ge, s_w, s_tmp, GRID_MARGIN);
     if (threadIdx.x == 0) D.cgp[ci] = g;
-}
-
-template <int B>
-__global__ __launch_bounds__(B) void k_grid(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
-                                            const int32_t* __restrict__ n_tiles, double cell_scale) {
-    __shared__ int s_start[BATCH_GRID_CELLS + 1];
-    __shared__ int s_cursor[BATCH_GRID_CELLS];
-    __shared__ int s_entries[BATCH_GRID_ENTRIES];
-    __shared__ double s_w[B / 64];
-    __shared__ int s_tmp[B / 64];
-    grid_contact<B, BATCH_GRID_CELLS, BATCH_GRID_ENTRIES, false>(table, tiles, n_tiles, cell_scale, s_start, s_cursor,
-                                                                 s_entries, s_w, s_tmp);
-}
-
-template <int B>
-__global__ __launch_bounds__(B) void k_grid_big(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
-                                                const int32_t* __restrict__ n_tiles, double cell_scale) {
-    __shared__ int s_start[BIG_GRID_CELLS + 1];
-    __shared__ int s_cursor[BIG_GRID_CELLS];
-    __shared__ int s_entries[BIG_GRID_ENTRIES];
-    __shared__ double s_w[B / 64];
-    __shared__ int s_tmp[B / 64];
-    grid_contact<B, BIG_GRID_CELLS, BIG_GRID_ENTRIES, true>(table, tiles, n_tiles, cell_scale, s_start, s_cursor,
-                                                            s_entries, s_w, s_tmp);
 }
 
 // (2) node search, tiles of T contacting nodes of one contact of one
@@ -662,7 +612,6 @@ static void add_tiles(const Prepared& P, int i, int T, std::vector<int2>* out) {
         const ContactDesc& cd = P.cdesc[ci];
         if (cd.n_faces == 0) continue;
         out[L_GRID].push_back(make_int2(i, ci));
-        if (cd.n_faces > BATCH_GRID_LARGE) out[L_GRIDB].push_back(make_int2(i, ci));
         for (int q = 0; q < cd.n_nodes; q += CONTACT_TILE) out[L_SEARCH].push_back(make_int2(i, cd.node_begin + q));
     }
     for (int k = 0; k < (int)P.tn_node.size(); k += CONTACT_TILE) out[L_APPLY].push_back(make_int2(i, k));
@@ -718,9 +667,6 @@ static void launch_step(BatchImpl& bi, double cs, hipEvent_t* ev) {
     mark(K_GRID);
     if (bi.cap[L_GRID] > 0)
         hipLaunchKernelGGL(k_grid<B>, dim3(bi.cap[L_GRID]), dim3(B), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID, cs);
-    if (bi.cap[L_GRIDB] > 0)
-        hipLaunchKernelGGL(k_grid_big<B>, dim3(bi.cap[L_GRIDB]), dim3(B), 0, st, tab, bi.d_tiles[L_GRIDB].p,
-                           cnt + L_GRIDB, cs);
     mark(K_SEARCH);
     if (bi.cap[L_SEARCH] > 0)
         hipLaunchKernelGGL(k_search_tiles<CONTACT_TILE>, dim3(bi.cap[L_SEARCH]), dim3(CONTACT_TILE), 0, st, tab,

@@ -14,15 +14,10 @@
 
 namespace crush {
 
+// Smallest face-grid region of a contact (cells, entries); larger contacts
+// get 8 cells and 64 entries per face.
 constexpr int BATCH_GRID_CELLS = 1024;
 constexpr int BATCH_GRID_ENTRIES = 4096;
-// Contacts with more faces than this may outgrow the 1024-cell LDS grid:
-// their grids then move to a kernel with a 64 KB LDS grid (BIG_GRID_CELLS
-// cells, BIG_GRID_ENTRIES entries), and past that to a device-memory region
-// of 8 cells and 64 entries per face (at least the large LDS capacities).
-constexpr int BATCH_GRID_LARGE = 400;
-constexpr int BIG_GRID_CELLS = 2048;
-constexpr int BIG_GRID_ENTRIES = 12000;
 constexpr int NODE_TILE = 256;
 constexpr int CONTACT_TILE = 64;
 // Contact grids are kept over steps: faces are binned grown by this
@@ -44,7 +39,7 @@ struct ImageOffsets {
 };
 
 // Active lists: the instance list and the tile lists of the tile kernels.
-enum { L_INST, L_HC, L_EL, L_NODE, L_FACE, L_GRID, L_SEARCH, L_APPLY, L_GRIDB, N_LISTS };
+enum { L_INST, L_HC, L_EL, L_NODE, L_FACE, L_GRID, L_SEARCH, L_APPLY, N_LISTS };
 // Kernels of one step, in launch order.
 enum { K_END, K_GRID, K_SEARCH, K_FACEFORCE, K_APPLY, K_HALF, K_FORCE_HC, K_FORCE_EL, K_GATHER, N_KERNELS };
 

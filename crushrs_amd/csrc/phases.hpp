@@ -188,10 +188,7 @@ struct FaceGrid {
     double h;
     int dim[3];
     int overflow;
-    // batched path, contacts with more than BATCH_GRID_LARGE faces: where the
-    // grid is built, 0 small LDS grid, 1 large LDS grid, 2 device memory;
-    // raised when a grid overflows and kept at later steps
-    int mode;
+    int reserved;
     int built;      // the grid arrays hold a grid of these parameters
     double margin;  // face boxes were binned grown by this much (batched path)
 };
@@ -681,7 +678,7 @@ __device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, dou
             h *= 1.25;
         }
         g.h = h;
-        g.mode = 0;
+        g.reserved = 0;
         g.built = 1;
         g.margin = m;
         g.overflow = (nf == 0 || cells > MAXC || !(isfinite(g.glo[0]) && isfinite(ghi[0]))) ? 1 : 0;
