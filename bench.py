@@ -66,8 +66,8 @@ B_HC, B_EL, B_NODE_FORCE, B_NODE_UPDATE = 776, 576, 36, 76
 # HBM bytes the honeycomb force kernel actually moves per element-step,
 # from rocprofv3 FETCH_SIZE and WRITE_SIZE counters on a batched sweep,
 # each calibrated against a 1 GiB device copy (FETCH_SIZE x 2.0,
-# WRITE_SIZE x 1.0): 258 B read, 201 B written.
-TRAFFIC_HC_READ, TRAFFIC_HC_WRITE = 258.0, 201.0
+# WRITE_SIZE x 1.0): 266 B read, 200 B written.
+TRAFFIC_HC_READ, TRAFFIC_HC_WRITE = 266.0, 200.0
 B_DT_ELEM, B_DT_NODE = 36, 24
 KERNELS = ("k_end", "k_grid", "k_search_tiles", "k_faceforce_tiles", "k_apply_tiles", "k_half_tiles",
            "k_force_hc", "k_force_el", "k_gather_tiles")
