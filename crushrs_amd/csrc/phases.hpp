@@ -33,6 +33,16 @@ typedef unsigned u32x3 __attribute__((ext_vector_type(3)));
 struct Float3 {
     float x, y, z;
 };
+struct Double3 {
+    double x, y, z;
+};
+// Nodal vector n of an interleaved xyz array as one 24-byte access.
+__device__ __forceinline__ Double3 ld_d3(const double* a, int n) {
+    return *reinterpret_cast<const Double3*>(a + 3 * (size_t)n);
+}
+__device__ __forceinline__ void st_d3(double* a, int n, const Double3& v) {
+    *reinterpret_cast<Double3*>(a + 3 * (size_t)n) = v;
+}
 // Three consecutive floats in one 12-byte store.
 __device__ __forceinline__ void st_f32x3(rsrc_t r, float a, float b, float c, unsigned voff, unsigned soff) {
     u32x3 v;
@@ -365,18 +375,22 @@ __device__ __forceinline__ bool half_node(const InstDev& D, int n, double dt) {
         return true;
     }
     const double im = D.inv_mass[n];
+    // each nodal vector moves as one 24-byte access (16 + 8 bytes)
+    const Double3 fi = ld_d3(D.f_int, n), v = ld_d3(D.v, n), u = ld_d3(D.u, n);
+    const Double3 fe = (fl & NODE_CONTACT) ? ld_d3(D.f_ext, n) : Double3{0.0, 0.0, 0.0};
+    const double fid[3] = {fi.x, fi.y, fi.z}, fed[3] = {fe.x, fe.y, fe.z};
+    const double vd[3] = {v.x, v.y, v.z}, ud[3] = {u.x, u.y, u.z};
+    double vh[3], un[3];
     bool ok = true;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const size_t k = 3 * (size_t)n + d;
-        const double fx = (fl & NODE_CONTACT) ? D.f_ext[k] : 0.0;
-        const double a = (fx - D.f_int[k]) * im;
-        const double vh = D.v[k] + 0.5 * dt * a;
-        D.v[k] = vh;
-        const double un = D.u[k] + dt * vh;
-        D.u[k] = un;
-        ok = ok && isfinite(un);
+        const double a = (fed[d] - fid[d]) * im;
+        vh[d] = vd[d] + 0.5 * dt * a;
+        un[d] = ud[d] + dt * vh[d];
+        ok = ok && isfinite(un[d]);
     }
+    st_d3(D.v, n, Double3{vh[0], vh[1], vh[2]});
+    st_d3(D.u, n, Double3{un[0], un[1], un[2]});
     return ok;
 }
 
@@ -546,17 +560,23 @@ __device__ __forceinline__ bool gather_node(const InstDev& D, int n, double dt, 
     const double im = D.inv_mass[n];
     const uint8_t fl = D.fixed[n];
     const bool fx = (fl & NODE_FIXED) != 0;
+    const Double3 v = ld_d3(D.v, n);
+    const Double3 fe = (fl & NODE_CONTACT) ? ld_d3(D.f_ext, n) : Double3{0.0, 0.0, 0.0};
+    const double vd[3] = {v.x, v.y, v.z}, fed[3] = {fe.x, fe.y, fe.z};
+    double vn[3];
     bool ok = true;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const size_t k = 3 * (size_t)n + d;
-        D.f_int[k] = f[d];
-        const double fe = (fl & NODE_CONTACT) ? D.f_ext[k] : 0.0;
-        const double a = (fe - f[d]) * im;
-        double vn = (D.v[k] + 0.5 * dt * a) * damp;
-        if (fx) vn = 0.0;
-        D.v[k] = vn;
-        ok = ok && isfinite(vn) && (!CHECK_U || isfinite(D.u[k]));
+        const double a = (fed[d] - f[d]) * im;
+        vn[d] = (vd[d] + 0.5 * dt * a) * damp;
+        if (fx) vn[d] = 0.0;
+        ok = ok && isfinite(vn[d]);
+    }
+    st_d3(D.f_int, n, Double3{f[0], f[1], f[2]});
+    st_d3(D.v, n, Double3{vn[0], vn[1], vn[2]});
+    if (CHECK_U) {
+        const Double3 u = ld_d3(D.u, n);
+        ok = ok && isfinite(u.x) && isfinite(u.y) && isfinite(u.z);
     }
     return ok;
 }
