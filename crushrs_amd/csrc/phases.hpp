@@ -30,6 +30,16 @@ __device__ __forceinline__ void st_f32(rsrc_t r, float v, unsigned voff, unsigne
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, voff, soff, 0);
 }
 typedef unsigned u32x3 __attribute__((ext_vector_type(3)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+// Three consecutive doubles in two loads (16 + 8 bytes).
+__device__ __forceinline__ void ld_f64x3(rsrc_t r, unsigned voff, double* x) {
+    const u32x4 a = __builtin_amdgcn_raw_buffer_load_b128(r, voff, 0, 0);
+    const u32x2 b = __builtin_amdgcn_raw_buffer_load_b64(r, voff + 16u, 0, 0);
+    x[0] = __builtin_bit_cast(double, (unsigned long long)a.x | ((unsigned long long)a.y << 32));
+    x[1] = __builtin_bit_cast(double, (unsigned long long)a.z | ((unsigned long long)a.w << 32));
+    x[2] = __builtin_bit_cast(double, b);
+}
 struct Float3 {
     float x, y, z;
 };
@@ -42,6 +52,14 @@ __device__ __forceinline__ Double3 ld_d3(const double* a, int n) {
 }
 __device__ __forceinline__ void st_d3(double* a, int n, const Double3& v) {
     *reinterpret_cast<Double3*>(a + 3 * (size_t)n) = v;
+}
+// Current position X + u of node n.
+__device__ __forceinline__ void pos3(const double* X, const double* u, int n, double* x) {
+#pragma clang fp contract(off)
+    const Double3 a = ld_d3(X, n), b = ld_d3(u, n);
+    x[0] = a.x + b.x;
+    x[1] = a.y + b.y;
+    x[2] = a.z + b.z;
 }
 // Three consecutive floats in one 12-byte store.
 __device__ __forceinline__ void st_f32x3(rsrc_t r, float a, float b, float c, unsigned voff, unsigned soff) {
@@ -148,8 +166,7 @@ __device__ __forceinline__ void dt_element(const InstDev& D, int s, double dt_fl
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int n = D.conn[(size_t)i * D.E + s];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) p[i][d] = D.X[3 * (size_t)n + d] + D.u[3 * (size_t)n + d];
+            pos3(D.X, D.u, n, p[i]);
         }
         double l, vol;
         char_length_d(p, &l, &vol);
@@ -183,8 +200,7 @@ __device__ __forceinline__ void contact_face(const InstDev& D, int f) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int n = D.cf_nodes[(size_t)k * D.F + f];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) x[k][d] = D.X[3 * (size_t)n + d] + D.u[3 * (size_t)n + d];
+        pos3(D.X, D.u, n, x[k]);
     }
     double g[FG_BLO];
     face_geometry(x, D.cdesc[D.cf_contact[f]].max_distance, g);
@@ -219,8 +235,7 @@ __device__ __forceinline__ int contact_node(const InstDev& D, const ContactDesc&
     const int node = D.cn_node[gk];
     const int F = D.F, fb = cd.face_begin;
     double p[3];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) p[d] = D.X[3 * (size_t)node + d] + D.u[3 * (size_t)node + d];
+    pos3(D.X, D.u, node, p);
     int best = -1;
     double best_sd = 0.0;
     if (g.overflow) {
@@ -357,8 +372,7 @@ __device__ __forceinline__ void contact_apply_node(const InstDev& D, int i) {
         }
     }
     const int n = D.tn_node[i];
-#pragma unroll
-    for (int d = 0; d < 3; ++d) D.f_ext[3 * (size_t)n + d] = s[d];
+    st_d3(D.f_ext, n, Double3{s[0], s[1], s[2]});
 }
 
 // First half step of node n (src/solver.rs:313-323); v holds v_1/2 after.
@@ -428,7 +442,9 @@ __device__ __forceinline__ int force_element(const InstDev& D, int s) {
         for (int k = 0; k < 25; ++k) st[k] = ld_f32(rst, vs, k * nh * 4u);
     }
     const unsigned vn0 = (unsigned)ld_i32(rconn, vs, 0) * 24u;
-    const double u0x = ld_f64(ru, vn0, 0), u0y = ld_f64(ru, vn0 + 8u, 0), u0z = ld_f64(ru, vn0 + 16u, 0);
+    double u0[3];
+    ld_f64x3(ru, vn0, u0);
+    const double u0x = u0[0], u0y = u0[1], u0z = u0[2];
     float hd[9], q[12];
 #pragma unroll
     for (int k = 0; k < 9; ++k) hd[k] = 0.0f;
@@ -437,9 +453,11 @@ __device__ __forceinline__ int force_element(const InstDev& D, int s) {
 #pragma unroll
     for (int i = 1; i < 8; ++i) {
         const unsigned vn = (unsigned)ld_i32(rconn, vs, i * E * 4u) * 24u;
-        const float d0 = (float)(ld_f64(ru, vn, 0) - u0x);
-        const float d1 = (float)(ld_f64(ru, vn + 8u, 0) - u0y);
-        const float d2 = (float)(ld_f64(ru, vn + 16u, 0) - u0z);
+        double ui[3];
+        ld_f64x3(ru, vn, ui);
+        const float d0 = (float)(ui[0] - u0x);
+        const float d1 = (float)(ui[1] - u0y);
+        const float d2 = (float)(ui[2] - u0z);
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
             const float g = ld_f32(rgrad, vc, (3 * i + b) * C * 4u);
