@@ -622,10 +622,23 @@ void rebuild_lists(BatchImpl& bi) {
     const int n = (int)bi.prep.size(), T = force_tile(bi.opt);
     std::vector<int32_t> list;
     std::vector<int2> tiles[N_LISTS];
+    int max_faces = 0;
     for (int i = 0; i < n; ++i) {
         if (bi.host_done[i]) continue;
         list.push_back(i);
         add_tiles(bi.prep[i], i, T, tiles);
+        for (const ContactDesc& cd : bi.prep[i].cdesc)
+            if (!cd.all_static) max_faces = std::max(max_faces, cd.n_faces);
+    }
+    // one k_grid thread per face of the largest contact whose faces move,
+    // from one wave up to sixteen (larger contacts loop); a server (headroom
+    // above 1) only grows it, so arrivals and retirements do not force
+    // captures back and forth
+    int gb = 64;
+    while (gb < 1024 && gb < max_faces) gb *= 2;
+    if (gb > bi.grid_block || (gb < bi.grid_block && bi.headroom <= 1.0)) {
+        bi.grid_block = gb;
+        bi.need_capture = true;
     }
     // grow the list buffers (a graph captured with the old ones is stale)
     auto need = [&](auto& buf, size_t want) {
@@ -665,8 +678,19 @@ static void launch_step(BatchImpl& bi, double cs, hipEvent_t* ev) {
     hipLaunchKernelGGL(k_end<B>, dim3(std::max(bi.cap[L_INST], 1)), dim3(B), 0, st, tab, bi.d_list.p, cnt + L_INST,
                        T, bi.done_flag.p, bi.n_done.p);
     mark(K_GRID);
-    if (bi.cap[L_GRID] > 0)
-        hipLaunchKernelGGL(k_grid<B>, dim3(bi.cap[L_GRID]), dim3(B), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID, cs);
+    if (bi.cap[L_GRID] > 0) {
+        const dim3 g(bi.cap[L_GRID]);
+        if (bi.grid_block == 64)
+            hipLaunchKernelGGL(k_grid<64>, g, dim3(64), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID, cs);
+        else if (bi.grid_block == 128)
+            hipLaunchKernelGGL(k_grid<128>, g, dim3(128), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID, cs);
+        else if (bi.grid_block == 256)
+            hipLaunchKernelGGL(k_grid<256>, g, dim3(256), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID, cs);
+        else if (bi.grid_block == 512)
+            hipLaunchKernelGGL(k_grid<512>, g, dim3(512), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID, cs);
+        else
+            hipLaunchKernelGGL(k_grid<1024>, g, dim3(1024), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID, cs);
+    }
     mark(K_SEARCH);
     if (bi.cap[L_SEARCH] > 0)
         hipLaunchKernelGGL(k_search_tiles<CONTACT_TILE>, dim3(bi.cap[L_SEARCH]), dim3(CONTACT_TILE), 0, st, tab,

@@ -61,6 +61,7 @@ struct BatchImpl {
     int cap[N_LISTS] = {};  // grids of the captured graph
     double headroom = 1.0;  // captured grids / current lists (server: > 1)
     bool need_capture = false;
+    int grid_block = 64;    // k_grid workgroup: the largest moving contact's faces, in waves
     bool fresh = true;  // no step taken since creation or reset
     DevBuf<BatchSummary> summary;
     std::vector<BatchSummary> host_summary;
