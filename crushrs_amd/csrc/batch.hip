@@ -141,6 +141,7 @@ ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     o.cf_nodes = b.put(P.cf_nodes);
     o.cf_contact = b.put(P.cf_contact);
     o.cf_static = b.put(P.cf_static);
+    o.cf_level = b.zeros((size_t)P.n_cfaces);
     o.tn_node = b.put(P.tn_node);
     o.tn_ptr = b.put(P.tn_ptr);
     o.tn_ent = b.put(P.tn_ent);
@@ -233,6 +234,7 @@ InstDev make_inst(const Prepared& P, const ImageOffsets& o, uintptr_t base) {
     I.cf_nodes = at<int32_t>(base, o.cf_nodes);
     I.cf_contact = at<int32_t>(base, o.cf_contact);
     I.cf_static = at<uint8_t>(base, o.cf_static);
+    I.cf_level = at<uint8_t>(base, o.cf_level);
     I.tn_node = at<int32_t>(base, o.tn_node);
     I.tn_ptr = at<int32_t>(base, o.tn_ptr);
     I.tn_ent = at<int32_t>(base, o.tn_ent);
@@ -413,6 +415,7 @@ __global__ __launch_bounds__(B) void k_grid(const InstDev* __restrict__ table, c
     for (int f = threadIdx.x; f < cd.n_faces; f += B) contact_face(D, cd.face_begin + f);
     __syncthreads();
     if (reusable && grid_still_valid<B>(D, cd, prev.margin)) return;
+    __syncthreads();  // the margin levels the check raised
     int32_t* gs = D.cgrid + cd.grid_off;
     int32_t* ge = gs + cd.grid_cells + 1;
     const FaceGrid g = build_face_grid<B>(D, cd, cell_scale, cd.grid_cells, cd.grid_entries, gs,

@@ -111,6 +111,7 @@ struct InstDev {
     const int32_t* cf_nodes;
     const int32_t* cf_contact;
     const uint8_t* cf_static;
+    uint8_t* cf_level;  // batched path: kept-grid margin of the face is margin x 2^level
     const int32_t* tn_node;
     const int32_t* tn_ptr;
     const int32_t* tn_ent;
@@ -665,16 +666,20 @@ __device__ void block_scan(int* s, int n, int* s_tmp) {
 // MAXC and MAXE are their capacities.
 //
 // margin_frac > 0 (the batched path, which keeps a grid over steps): every
-// face is binned with its candidate box grown by margin = margin_frac x the
-// largest face extent, and that box is recorded (FG_BLO, FG_BHI); the grid
-// serves later steps as long as every face's box stays inside its recorded
-// box grown by the margin (grid_still_valid).
+// face is binned with its candidate box grown by its margin, margin_frac x
+// the largest face extent x 2^level (cf_level), and that box is recorded
+// (FG_BLO, FG_BHI); the grid serves later steps as long as every face's box
+// stays inside its recorded box grown by its margin (grid_still_valid).  In
+// large contacts a face that leaves it has its level raised (up to 3), so a
+// face whose candidate box swings as it crushes stops forcing rebuilds while
+// the others keep tight bins.
 template <int B>
 __device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, double cell_scale, const int MAXC,
                                     const int MAXE, int* s_start, int* s_cursor, int* s_entries, double* s_w,
                                     int* s_tmp, double margin_frac = 0.0) {
     const int t = threadIdx.x, F = D.F, fb = cd.face_begin, nf = cd.n_faces;
-    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY}, sz = 0.0;
+    const bool kept = margin_frac > 0.0;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY}, sz = 0.0, grow = 1.0;
     for (int f = t; f < nf; f += B) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
@@ -682,6 +687,7 @@ __device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, dou
             hi[d] = fmax(hi[d], D.fgeo[(size_t)(FG_CHI + d) * F + fb + f]);
         }
         sz = fmax(sz, D.fgeo[(size_t)FG_SZ * F + fb + f]);
+        if (kept) grow = fmax(grow, (double)(1 << D.cf_level[fb + f]));
     }
     FaceGrid g;
     double ghi[3];
@@ -691,11 +697,14 @@ __device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, dou
         ghi[d] = block_reduce<B>(hi[d], 2, s_w);
     }
     sz = block_reduce<B>(sz, 2, s_w);
+    grow = block_reduce<B>(grow, 2, s_w);
     const double m = margin_frac * sz;
+    // per-face margin: m x 2^level on the kept grids, 0 otherwise
+    auto face_margin = [&](int f) { return kept ? m * (double)(1 << D.cf_level[fb + f]) : 0.0; };
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        g.glo[d] -= m;
-        ghi[d] += m;
+        g.glo[d] -= m * grow;
+        ghi[d] += m * grow;
     }
     double h = fmax(sz * cell_scale, 1e-9);
     // cell size: grown until the grid has at most MAXC cells; when the
@@ -734,9 +743,10 @@ __device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, dou
                     fhi[d] = D.fgeo[(size_t)(FG_CHI + d) * F + fb + f];
                 }
                 if (!(flo[0] <= fhi[0] && flo[1] <= fhi[1] && flo[2] <= fhi[2])) continue;
-                const int a0 = grid_cell(g, flo[0] - m, 0), a1 = grid_cell(g, fhi[0] + m, 0);
-                const int b0 = grid_cell(g, flo[1] - m, 1), b1 = grid_cell(g, fhi[1] + m, 1);
-                const int c0 = grid_cell(g, flo[2] - m, 2), c1 = grid_cell(g, fhi[2] + m, 2);
+                const double mf = face_margin(f);
+                const int a0 = grid_cell(g, flo[0] - mf, 0), a1 = grid_cell(g, fhi[0] + mf, 0);
+                const int b0 = grid_cell(g, flo[1] - mf, 1), b1 = grid_cell(g, fhi[1] + mf, 1);
+                const int c0 = grid_cell(g, flo[2] - mf, 2), c1 = grid_cell(g, fhi[2] + mf, 2);
                 for (int kz = c0; kz <= c1; ++kz)
                     for (int ky = b0; ky <= b1; ++ky)
                         for (int kx = a0; kx <= a1; ++kx) atomicAdd(&s_start[(kz * dy + ky) * dx + kx], 1);
@@ -766,9 +776,10 @@ __device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, dou
                     D.fgeo[(size_t)(FG_BHI + d) * F + fb + f] = fhi[d];
                 }
             if (!(flo[0] <= fhi[0] && flo[1] <= fhi[1] && flo[2] <= fhi[2])) continue;
-            const int a0 = grid_cell(g, flo[0] - m, 0), a1 = grid_cell(g, fhi[0] + m, 0);
-            const int b0 = grid_cell(g, flo[1] - m, 1), b1 = grid_cell(g, fhi[1] + m, 1);
-            const int c0 = grid_cell(g, flo[2] - m, 2), c1 = grid_cell(g, fhi[2] + m, 2);
+            const double mf = face_margin(f);
+            const int a0 = grid_cell(g, flo[0] - mf, 0), a1 = grid_cell(g, fhi[0] + mf, 0);
+            const int b0 = grid_cell(g, flo[1] - mf, 1), b1 = grid_cell(g, fhi[1] + mf, 1);
+            const int c0 = grid_cell(g, flo[2] - mf, 2), c1 = grid_cell(g, fhi[2] + mf, 2);
             for (int kz = c0; kz <= c1; ++kz)
                 for (int ky = b0; ky <= b1; ++ky)
                     for (int kx = a0; kx <= a1; ++kx) s_entries[atomicAdd(&s_cursor[(kz * dy + ky) * dx + kx], 1)] = f;
@@ -780,23 +791,32 @@ __device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, dou
 
 // Whether the grid built with margin `m` still serves this step: every
 // face whose candidate box is valid has it inside the box recorded at the
-// build grown by m (then every cell a node in that box maps to lists the
-// face).  Every thread of the block calls it.
+// build grown by its margin, m x 2^level (then every cell a node in that
+// box maps to lists the face).  In contacts of more than LEVEL_MIN_FACES
+// faces, where a rebuild costs several times the per-step face pass, a face
+// outside has its level raised for the rebuild that follows.  Every thread
+// of the block calls it.
+constexpr int LEVEL_MIN_FACES = 256;
 template <int B>
 __device__ bool grid_still_valid(const InstDev& D, const ContactDesc& cd, double m) {
     const int F = D.F, fb = cd.face_begin;
     int stale = 0;
     for (int f = threadIdx.x; f < cd.n_faces; f += B) {
         if (D.cf_static[fb + f]) continue;
+        const int level = D.cf_level[fb + f];
+        const double mf = m * (double)(1 << level);
         bool valid = true, inside = true;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             const double lo = D.fgeo[(size_t)(FG_CLO + d) * F + fb + f], hi = D.fgeo[(size_t)(FG_CHI + d) * F + fb + f];
             valid = valid && lo <= hi;
-            inside = inside && lo >= D.fgeo[(size_t)(FG_BLO + d) * F + fb + f] - m &&
-                     hi <= D.fgeo[(size_t)(FG_BHI + d) * F + fb + f] + m;
+            inside = inside && lo >= D.fgeo[(size_t)(FG_BLO + d) * F + fb + f] - mf &&
+                     hi <= D.fgeo[(size_t)(FG_BHI + d) * F + fb + f] + mf;
         }
-        if (valid && !inside) stale = 1;
+        if (valid && !inside) {
+            stale = 1;
+            if (level < 3 && cd.n_faces > LEVEL_MIN_FACES) D.cf_level[fb + f] = (uint8_t)(level + 1);
+        }
     }
     return !__syncthreads_or(stale);
 }
