@@ -418,8 +418,21 @@ __global__ __launch_bounds__(B) void k_grid(const InstDev* __restrict__ table, c
     __syncthreads();  // the margin levels the check raised
     int32_t* gs = D.cgrid + cd.grid_off;
     int32_t* ge = gs + cd.grid_cells + 1;
-    const FaceGrid g = build_face_grid<B>(D, cd, cell_scale, cd.grid_cells, cd.grid_entries, gs,
-                                          ge + cd.grid_entries, ge, s_w, s_tmp, GRID_MARGIN);
+    FaceGrid g;
+    if constexpr (B >= 512) {
+        // large contacts: cell counts, scan and fill cursors in LDS (the
+        // fill's returning atomics are the rebuild's latency chain), the
+        // starts copied out once
+        __shared__ int l_start[LDS_GRID_CELLS + 1];
+        __shared__ int l_cursor[LDS_GRID_CELLS];
+        g = build_face_grid<B>(D, cd, cell_scale, min(cd.grid_cells, LDS_GRID_CELLS), cd.grid_entries, l_start,
+                               l_cursor, ge, s_w, s_tmp, GRID_MARGIN);
+        const int nc = g.overflow ? 1 : g.dim[0] * g.dim[1] * g.dim[2];
+        for (int i = threadIdx.x; i <= nc; i += B) gs[i] = l_start[i];
+    } else {
+        g = build_face_grid<B>(D, cd, cell_scale, cd.grid_cells, cd.grid_entries, gs, ge + cd.grid_entries, ge, s_w,
+                               s_tmp, GRID_MARGIN);
+    }
     if (threadIdx.x == 0) D.cgp[ci] = g;
 }
 

@@ -18,6 +18,8 @@ namespace crush {
 // get 8 cells and 64 entries per face.
 constexpr int BATCH_GRID_CELLS = 1024;
 constexpr int BATCH_GRID_ENTRIES = 4096;
+// Cells of a grid built in LDS (k_grid workgroups of 512 or more threads)
+constexpr int LDS_GRID_CELLS = 4096;
 constexpr int NODE_TILE = 256;
 constexpr int CONTACT_TILE = 64;
 // Contact grids are kept over steps: faces are binned grown by this
