@@ -437,10 +437,12 @@ __global__ __launch_bounds__(B) void k_grid(const InstDev* __restrict__ table, c
 }
 
 // (2) node search, tiles of T contacting nodes of one contact of one
-// instance; penetrating nodes of pair contacts are counted per instance.
-template <int T>
-__global__ __launch_bounds__(T) void k_search_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
-                                                    const int32_t* __restrict__ n_tiles) {
+// instance, L lanes per node; penetrating nodes of pair contacts are
+// counted per instance.
+template <int T, int L>
+__global__ __launch_bounds__(T * L) void k_search_tiles(const InstDev* __restrict__ table,
+                                                        const int2* __restrict__ tiles,
+                                                        const int32_t* __restrict__ n_tiles) {
     if ((int)blockIdx.x >= *n_tiles) return;
     const int2 tile = tiles[blockIdx.x];
     const InstDev& D = table[tile.x];
@@ -449,12 +451,12 @@ __global__ __launch_bounds__(T) void k_search_tiles(const InstDev* __restrict__ 
     int ci = 0;
     while (ci + 1 < D.n_contacts && D.cdesc[ci + 1].node_begin <= tile.y) ++ci;
     const ContactDesc cd = D.cdesc[ci];
-    const int q = tile.y - cd.node_begin + (int)threadIdx.x;
+    const int q = tile.y - cd.node_begin + (int)threadIdx.x / L;
     int hit = 0;
     if (q < cd.n_nodes) {
         const FaceGrid g = D.cgp[ci];
         const int32_t* gs = D.cgrid + cd.grid_off;
-        hit = contact_node(D, cd, q, g, gs, gs + cd.grid_cells + 1, step_dt(c));
+        hit = contact_node<L>(D, cd, q, g, gs, gs + cd.grid_cells + 1, step_dt(c), (int)threadIdx.x % L);
     }
     const unsigned long long m = __ballot(hit != 0 && cd.pair);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(&c.pair_hits, (int)__popcll(m));
@@ -656,6 +658,13 @@ void rebuild_lists(BatchImpl& bi) {
         bi.grid_block = gb;
         bi.need_capture = true;
     }
+    // a batch with few search tiles gives each contacting node 4 lanes (the
+    // server keeps one: its lists change with every arrival)
+    const int sl = (bi.headroom <= 1.0 && (int)tiles[L_SEARCH].size() < SEARCH_SPLIT_TILES) ? 4 : 1;
+    if (sl != bi.search_lanes) {
+        bi.search_lanes = sl;
+        bi.need_capture = true;
+    }
     // grow the list buffers (a graph captured with the old ones is stale)
     auto need = [&](auto& buf, size_t want) {
         if (want > buf.n) {
@@ -709,8 +718,12 @@ static void launch_step(BatchImpl& bi, double cs, hipEvent_t* ev) {
     }
     mark(K_SEARCH);
     if (bi.cap[L_SEARCH] > 0)
-        hipLaunchKernelGGL(k_search_tiles<CONTACT_TILE>, dim3(bi.cap[L_SEARCH]), dim3(CONTACT_TILE), 0, st, tab,
-                           bi.d_tiles[L_SEARCH].p, cnt + L_SEARCH);
+        if (bi.search_lanes == 4)
+            hipLaunchKernelGGL((k_search_tiles<CONTACT_TILE, 4>), dim3(bi.cap[L_SEARCH]), dim3(4 * CONTACT_TILE), 0, st,
+                               tab, bi.d_tiles[L_SEARCH].p, cnt + L_SEARCH);
+        else
+            hipLaunchKernelGGL((k_search_tiles<CONTACT_TILE, 1>), dim3(bi.cap[L_SEARCH]), dim3(CONTACT_TILE), 0, st,
+                               tab, bi.d_tiles[L_SEARCH].p, cnt + L_SEARCH);
     mark(K_FACEFORCE);
     if (bi.cap[L_FACE] > 0)
         hipLaunchKernelGGL(k_faceforce_tiles<CONTACT_TILE>, dim3(bi.cap[L_FACE]), dim3(CONTACT_TILE), 0, st, tab,

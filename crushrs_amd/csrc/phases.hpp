@@ -228,9 +228,13 @@ __device__ __forceinline__ int grid_cell(const FaceGrid& g, double x, int d) {
 // Penalty force of contacting node gk (src/contact.rs:113-157): the
 // lowest-index accepting face among the grid candidates (or, on grid
 // overflow, the first accepting face in order).  Records the hit and
-// returns 1 when the node penetrates.
+// returns 1 when the node penetrates.  With L > 1, L consecutive lanes of a
+// wave share the node: lane `sub` tests every L-th group of candidates, the
+// lanes' lowest accepting faces are reduced to the lowest, and lane 0 of the
+// group records the hit.
+template <int L = 1>
 __device__ __forceinline__ int contact_node(const InstDev& D, const ContactDesc& cd, int k, const FaceGrid& g,
-                                            const int* s_start, const int* s_entries, double dt) {
+                                            const int* s_start, const int* s_entries, double dt, int sub = 0) {
 #pragma clang fp contract(off)
     const int gk = cd.node_begin + k;
     const int node = D.cn_node[gk];
@@ -240,7 +244,7 @@ __device__ __forceinline__ int contact_node(const InstDev& D, const ContactDesc&
     int best = -1;
     double best_sd = 0.0;
     if (g.overflow) {
-        for (int f = 0; f < cd.n_faces; ++f) {
+        for (int f = sub; f < cd.n_faces; f += L) {
             double sd;
             if (face_accepts(D.fgeo, F, fb + f, p, cd.max_distance, &sd)) {
                 best = f;
@@ -263,7 +267,7 @@ __device__ __forceinline__ int contact_node(const InstDev& D, const ContactDesc&
             // (a superset of what the face can accept) skips the test
             const int cell = (ci[2] * g.dim[1] + ci[1]) * g.dim[0] + ci[0];
             const int qe = s_start[cell + 1];
-            for (int q0 = s_start[cell]; q0 < qe; q0 += 4) {
+            for (int q0 = s_start[cell] + 4 * sub; q0 < qe; q0 += 4 * L) {
                 int fs[4];
 #pragma unroll
                 for (int j = 0; j < 4; ++j) fs[j] = q0 + j < qe ? s_entries[q0 + j] : -1;
@@ -290,6 +294,19 @@ __device__ __forceinline__ int contact_node(const InstDev& D, const ContactDesc&
                 }
             }
         }
+    }
+    if (L > 1) {
+        // lowest accepting face of the group, with its signed distance
+#pragma unroll
+        for (int off = 1; off < L; off <<= 1) {
+            const int ob = __shfl_xor(best, off, 64);
+            const double osd = __shfl_xor(best_sd, off, 64);
+            if (ob >= 0 && (best < 0 || ob < best)) {
+                best = ob;
+                best_sd = osd;
+            }
+        }
+        if (sub != 0) return 0;
     }
     if (best < 0) {
         D.rec_face[gk] = -1;
