@@ -412,10 +412,16 @@ __global__ __launch_bounds__(B) void k_grid(const InstDev* __restrict__ table, c
     const FaceGrid prev = D.cgp[ci];
     const bool reusable = prev.built && !prev.overflow;
     if (reusable && cd.all_static) return;
-    for (int f = threadIdx.x; f < cd.n_faces; f += B) contact_face(D, cd.face_begin + f);
-    __syncthreads();
-    if (reusable && grid_still_valid<B>(D, cd, prev.margin)) return;
-    __syncthreads();  // the margin levels the check raised
+    if (large_moving_contact(cd)) {
+        // face pass and check done by k_facegeo_tiles, which marks the grid
+        // stale (FaceGrid::stale)
+        if (reusable && !prev.stale) return;
+    } else {
+        for (int f = threadIdx.x; f < cd.n_faces; f += B) contact_face(D, cd.face_begin + f);
+        __syncthreads();
+        if (reusable && grid_still_valid<B>(D, cd, prev.margin)) return;
+    }
+    __syncthreads();  // the face geometry and the margin levels
     int32_t* gs = D.cgrid + cd.grid_off;
     int32_t* ge = gs + cd.grid_cells + 1;
     FaceGrid g;
@@ -434,6 +440,29 @@ __global__ __launch_bounds__(B) void k_grid(const InstDev* __restrict__ table, c
                                s_tmp, GRID_MARGIN);
     }
     if (threadIdx.x == 0) D.cgp[ci] = g;
+}
+
+// (1a) face geometry of large moving contacts, tiles of T faces spread over
+// the GPU (one workgroup per contact would run their f64 face pass on one
+// CU each); tile.y = contact << 20 | first face in the contact.  Each face
+// is checked against its contact's kept grid, and a face outside it marks
+// the grid stale for k_grid.
+template <int T>
+__global__ __launch_bounds__(T) void k_facegeo_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
+                                                     const int32_t* __restrict__ n_tiles) {
+    if ((int)blockIdx.x >= *n_tiles) return;
+    const int2 tile = tiles[blockIdx.x];
+    const InstDev& D = table[tile.x];
+    if (!D.ctl->pending) return;
+    const int ci = tile.y >> 20;
+    const ContactDesc cd = D.cdesc[ci];
+    const int fl = (tile.y & 0xFFFFF) + (int)threadIdx.x;
+    if (fl >= cd.n_faces) return;
+    const int f = cd.face_begin + fl;
+    contact_face(D, f);
+    const FaceGrid prev = D.cgp[ci];
+    if (!prev.built || prev.overflow || D.cf_static[f]) return;
+    if (!kept_box_holds(D, cd, f, prev.margin)) atomicOr(&D.cgp[ci].stale, 1);
 }
 
 // (2) node search, tiles of T contacting nodes of one contact of one
@@ -630,6 +659,12 @@ static void add_tiles(const Prepared& P, int i, int T, std::vector<int2>* out) {
         const ContactDesc& cd = P.cdesc[ci];
         if (cd.n_faces == 0) continue;
         out[L_GRID].push_back(make_int2(i, ci));
+        if (large_moving_contact(cd)) {
+            if (ci >= (1 << 11) || cd.n_faces >= (1 << 20))
+                fail(CRUSH_E_INPUT, "contact %d of %d faces: beyond the face tiles' 2048 contacts / 2^20 faces", ci,
+                     cd.n_faces);
+            for (int f = 0; f < cd.n_faces; f += CONTACT_TILE) out[L_FGEO].push_back(make_int2(i, (ci << 20) | f));
+        }
         for (int q = 0; q < cd.n_nodes; q += CONTACT_TILE) out[L_SEARCH].push_back(make_int2(i, cd.node_begin + q));
     }
     for (int k = 0; k < (int)P.tn_node.size(); k += CONTACT_TILE) out[L_APPLY].push_back(make_int2(i, k));
@@ -703,6 +738,9 @@ static void launch_step(BatchImpl& bi, double cs, hipEvent_t* ev) {
     hipLaunchKernelGGL(k_end<B>, dim3(std::max(bi.cap[L_INST], 1)), dim3(B), 0, st, tab, bi.d_list.p, cnt + L_INST,
                        T, bi.done_flag.p, bi.n_done.p);
     mark(K_GRID);
+    if (bi.cap[L_FGEO] > 0)
+        hipLaunchKernelGGL(k_facegeo_tiles<CONTACT_TILE>, dim3(bi.cap[L_FGEO]), dim3(CONTACT_TILE), 0, st, tab,
+                           bi.d_tiles[L_FGEO].p, cnt + L_FGEO);
     if (bi.cap[L_GRID] > 0) {
         const dim3 g(bi.cap[L_GRID]);
         if (bi.grid_block == 64)

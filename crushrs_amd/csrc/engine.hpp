@@ -43,7 +43,7 @@ struct ImageOffsets {
 };
 
 // Active lists: the instance list and the tile lists of the tile kernels.
-enum { L_INST, L_HC, L_EL, L_NODE, L_FACE, L_GRID, L_SEARCH, L_APPLY, N_LISTS };
+enum { L_INST, L_HC, L_EL, L_NODE, L_FACE, L_GRID, L_SEARCH, L_APPLY, L_FGEO, N_LISTS };
 // Kernels of one step, in launch order.
 enum { K_END, K_GRID, K_SEARCH, K_FACEFORCE, K_APPLY, K_HALF, K_FORCE_HC, K_FORCE_EL, K_GATHER, N_KERNELS };
 

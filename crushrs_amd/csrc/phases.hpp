@@ -215,7 +215,7 @@ struct FaceGrid {
     double h;
     int dim[3];
     int overflow;
-    int reserved;
+    int stale;      // batched path, large contacts: a face left its kept box this step
     int built;      // the grid arrays hold a grid of these parameters
     double margin;  // face boxes were binned grown by this much (batched path)
 };
@@ -686,7 +686,7 @@ __device__ void block_scan(int* s, int n, int* s_tmp) {
 // face is binned with its candidate box grown by its margin, margin_frac x
 // the largest face extent x 2^level (cf_level), and that box is recorded
 // (FG_BLO, FG_BHI); the grid serves later steps as long as every face's box
-// stays inside its recorded box grown by its margin (grid_still_valid).  In
+// stays inside its recorded box grown by its margin (kept_box_holds).  In
 // large contacts a face that leaves it has its level raised (up to 3), so a
 // face whose candidate box swings as it crushes stops forcing rebuilds while
 // the others keep tight bins.
@@ -742,7 +742,7 @@ __device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, dou
             h *= 1.25;
         }
         g.h = h;
-        g.reserved = 0;
+        g.stale = 0;
         g.built = 1;
         g.margin = m;
         g.overflow = (nf == 0 || cells > MAXC || !(isfinite(g.glo[0]) && isfinite(ghi[0]))) ? 1 : 0;
@@ -806,34 +806,49 @@ __device__ FaceGrid build_face_grid(const InstDev& D, const ContactDesc& cd, dou
     return g;
 }
 
+// Contacts of more than LARGE_CONTACT_FACES faces (the batched path): a
+// rebuild costs several times the per-step face pass, so their faces get
+// margin levels, and the face pass runs in tiles over the whole GPU.
+constexpr int LARGE_CONTACT_FACES = 256;
+
+__host__ __device__ __forceinline__ bool large_moving_contact(const ContactDesc& cd) {
+    return cd.n_faces > LARGE_CONTACT_FACES && !cd.all_static;
+}
+
+// Whether face f (global index) of a contact with a kept grid of margin m
+// is still served by it: its candidate box, when valid, lies inside the box
+// recorded at the build grown by its margin, m x 2^level (then every cell a
+// node in that box maps to lists the face).  In large contacts a face
+// outside has its level raised for the rebuild that follows.
+__device__ __forceinline__ bool kept_box_holds(const InstDev& D, const ContactDesc& cd, int f, double m) {
+    const int F = D.F;
+    const int level = D.cf_level[f];
+    const double mf = m * (double)(1 << level);
+    bool valid = true, inside = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double lo = D.fgeo[(size_t)(FG_CLO + d) * F + f], hi = D.fgeo[(size_t)(FG_CHI + d) * F + f];
+        valid = valid && lo <= hi;
+        inside = inside && lo >= D.fgeo[(size_t)(FG_BLO + d) * F + f] - mf &&
+                 hi <= D.fgeo[(size_t)(FG_BHI + d) * F + f] + mf;
+    }
+    if (valid && !inside) {
+        if (level < 3 && cd.n_faces > LARGE_CONTACT_FACES) D.cf_level[f] = (uint8_t)(level + 1);
+        return false;
+    }
+    return true;
+}
+
 // Whether the grid built with margin `m` still serves this step: every
-// face whose candidate box is valid has it inside the box recorded at the
-// build grown by its margin, m x 2^level (then every cell a node in that
-// box maps to lists the face).  In contacts of more than LEVEL_MIN_FACES
-// faces, where a rebuild costs several times the per-step face pass, a face
-// outside has its level raised for the rebuild that follows.  Every thread
-// of the block calls it.
-constexpr int LEVEL_MIN_FACES = 256;
+// face that is not static passes kept_box_holds.  Every thread of the
+// block calls it.
 template <int B>
 __device__ bool grid_still_valid(const InstDev& D, const ContactDesc& cd, double m) {
-    const int F = D.F, fb = cd.face_begin;
+    const int fb = cd.face_begin;
     int stale = 0;
     for (int f = threadIdx.x; f < cd.n_faces; f += B) {
         if (D.cf_static[fb + f]) continue;
-        const int level = D.cf_level[fb + f];
-        const double mf = m * (double)(1 << level);
-        bool valid = true, inside = true;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            const double lo = D.fgeo[(size_t)(FG_CLO + d) * F + fb + f], hi = D.fgeo[(size_t)(FG_CHI + d) * F + fb + f];
-            valid = valid && lo <= hi;
-            inside = inside && lo >= D.fgeo[(size_t)(FG_BLO + d) * F + fb + f] - mf &&
-                     hi <= D.fgeo[(size_t)(FG_BHI + d) * F + fb + f] + mf;
-        }
-        if (valid && !inside) {
-            stale = 1;
-            if (level < 3 && cd.n_faces > LEVEL_MIN_FACES) D.cf_level[fb + f] = (uint8_t)(level + 1);
-        }
+        if (!kept_box_holds(D, cd, fb + f, m)) stale = 1;
     }
     return !__syncthreads_or(stale);
 }
