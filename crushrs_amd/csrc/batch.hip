@@ -2,11 +2,15 @@
 //
 // Each instance is one contiguous image in a device pool (its arrays plus a
 // control block) described by an InstDev pointer table.  One explicit step
-// of every active instance is nine launches, each over all instances:
+// of every active instance is nine launches (ten with a large contact),
+// each over all instances:
 //   k_end              one workgroup per instance: end of the previous step
 //                      and the stable step of the next (at check steps);
+//   k_facegeo_tiles    face geometry of contacts of more than 256 moving
+//                      faces, in tiles of faces;
 //   k_grid             one workgroup per contact of an instance: face
-//                      geometry and the face grid, stored to device memory;
+//                      geometry of the other contacts and the face grid,
+//                      stored to device memory;
 //   k_search_tiles     node search against the grid (tiles of contacting
 //                      nodes); k_faceforce_tiles face reactions (tiles of
 //                      faces); k_apply_tiles contact forces (tiles of
@@ -389,12 +393,13 @@ __global__ __launch_bounds__(B) void k_end(const InstDev* __restrict__ table, co
     if (t == 0) *D.ctl = *c;
 }
 
-// Contact (src/contact.rs:92-160), in four launches over all open steps.
-// (1) geometry of the faces of one contact of one instance, and its face
-// grid in the contact's device-memory region (8 cells and 64 entries per
-// face, at least BATCH_GRID_CELLS and BATCH_GRID_ENTRIES).  A grid is kept
-// over steps while every face stays inside its box at the build grown by
-// the grid's margin, so builds are rare and need no LDS.  The grid layout
+// Contact (src/contact.rs:92-160), in four or five launches over all open
+// steps.  (1) geometry of the faces of one contact of one instance (done by
+// k_facegeo_tiles for large contacts), and its face grid in the contact's
+// device-memory region (8 cells and 64 entries per face, at least
+// BATCH_GRID_CELLS and BATCH_GRID_ENTRIES).  A grid is kept over steps
+// while every face stays inside its box at the build grown by its margin,
+// so builds are rare; workgroups of 512 or more threads build in LDS.  The grid layout
 // never changes which face a node hits: the search keeps the lowest
 // accepting face index.
 template <int B>
