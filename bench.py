@@ -55,7 +55,8 @@ from crushrs_amd import workloads  # noqa: E402
 METRIC = "batched crash solves/s + p50/p99 latency vs plain GPU; Gelem-steps/s, %HBM"
 HBM_PEAK_GBS = 8000.0
 
-# Algorithmic bytes of one explicit step (f32 element data, f64 nodal data):
+# Algorithmic bytes of one explicit step in the reference's storage model
+# (f32 element data, f64 nodal data, geometry constants stored per element):
 # force pass 776 B per honeycomb element (connectivity 32, gradients/volume/
 # hourglass constants 540, state 100 read + 100 written, part/erosion 4),
 # 576 B per elastic element (no state), 36 B per node (gather X and u, write
@@ -63,12 +64,22 @@ HBM_PEAK_GBS = 8000.0
 # adaptive_check_steps steps 36 B per element and 24 B per node.  Contact is
 # not counted.
 B_HC, B_EL, B_NODE_FORCE, B_NODE_UPDATE = 776, 576, 36, 76
+# Bytes the honeycomb force kernel must move in this library's layout, where
+# the geometry constants are stored once per geometry class and stay in
+# cache: per element-step the connectivity 32, the class index 4, the
+# erosion flag 1, the state 100 read + 100 written and the per-corner forces
+# 96 written (333 B), plus each node of the honeycomb elements' displacement
+# once (24 B, f64 xyz).  This is the roofline's algorithmic figure.
+LAYOUT_HC_ELEM, LAYOUT_HC_NODE = 333, 24
 # HBM bytes the honeycomb force kernel actually moves per element-step,
 # from rocprofv3 FETCH_SIZE and WRITE_SIZE counters on a batched sweep,
 # each calibrated against a 1 GiB device copy (FETCH_SIZE x 2.0,
 # WRITE_SIZE x 1.0): 266 B read, 200 B written.
 TRAFFIC_HC_READ, TRAFFIC_HC_WRITE = 266.0, 200.0
 B_DT_ELEM, B_DT_NODE = 36, 24
+# Worker threads of one GPU's job on a shared host: the share the GPU box
+# gives one GPU (its scheduler runs one job per GPU with this many CPUs).
+CPU_SHARE = 16
 KERNELS = ("k_end", "k_grid", "k_search_tiles", "k_faceforce_tiles", "k_apply_tiles", "k_half_tiles",
            "k_force_hc", "k_force_el", "k_gather_tiles")
 
@@ -89,6 +100,43 @@ def cpu_info() -> str:
     return platform.processor() or "unknown"
 
 
+def host_cpus() -> dict:
+    """What the host offers this process: logical CPUs of the machine, CPUs
+    in the affinity mask, the cgroup CPU quota, physical cores and sockets,
+    and the worker count the CPU baselines use (every CPU of the process's
+    share: the affinity mask and quota, at most CPU_SHARE)."""
+    logical = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = logical
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    cores, sockets = set(), set()
+    try:
+        phys = core = None
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("physical id"):
+                phys = line.split(":", 1)[1].strip()
+            elif line.startswith("core id"):
+                core = line.split(":", 1)[1].strip()
+            elif not line.strip() and phys is not None:
+                sockets.add(phys)
+                cores.add((phys, core))
+                phys = core = None
+    except OSError:
+        pass
+    share = affinity if quota is None else min(affinity, max(1, int(quota)))
+    return {"cpu_model": cpu_info(), "logical_cpus": logical, "physical_cores": len(cores) or None,
+            "sockets": len(sockets) or None, "affinity_cpus": affinity, "cgroup_quota_cpus": quota,
+            "threads": max(1, min(share, CPU_SHARE))}
+
+
 def cpu_baseline(models, elem_steps_per_solve: float, budget_s: float = 20.0, sample_steps: int = 100,
                  path: str = "lanes"):
     """The reference's CPU path restated in C, timed on the host cores.
@@ -106,7 +154,8 @@ def cpu_baseline(models, elem_steps_per_solve: float, budget_s: float = 20.0, sa
     import ctypes as C
     import oracle_ffi as of
     lib = of.load()
-    cores = max(1, min(os.cpu_count() or 1, 16))
+    host = host_cpus()
+    cores = host["threads"]
     rng = np.random.default_rng(1)
     order = rng.permutation(len(models))
     t_end = time.perf_counter() + budget_s
@@ -137,6 +186,7 @@ def cpu_baseline(models, elem_steps_per_solve: float, budget_s: float = 20.0, sa
         res = list(ex.map(worker, range(cores)))
     rate = sum(r for r, _ in res)
     n = sum(k for _, k in res)
+    per_core = rate / cores
     what = ("AVX2 8-lane f32 restatement of the reference's default element path (kernel_simd.rs; "
             "elastic parts scalar f64), -O3 -mavx2 -mfma" if path == "lanes" else
             "f64 scalar restatement of the reference's solver (kernel.rs path)")
@@ -146,9 +196,14 @@ def cpu_baseline(models, elem_steps_per_solve: float, budget_s: float = 20.0, sa
         "cores": cores,
         "kind": "port",
         "sample": (f"{what}; one solve per thread, {n} sampled models x {sample_steps} steps on {cores} threads of "
-                   f"{cpu_info()}; {rate / 1e6:.2f} M element-steps/s converted with this workload's mean "
+                   f"{host['cpu_model']}; {rate / 1e6:.2f} M element-steps/s converted with this workload's mean "
                    f"{elem_steps_per_solve:.3g} element-steps per solve"),
         "gelem_steps_per_s": rate / 1e9,
+        "host": host,
+        # one solve per core scales linearly (independent solves, no shared
+        # data): the rate on every physical core of the machine, for scale
+        "all_cores_extrapolated": (per_core * (host["physical_cores"] or host["logical_cpus"])
+                                   / elem_steps_per_solve),
     }
 
 
@@ -195,9 +250,74 @@ def copy_probe(nbytes: int) -> None:
     print(json.dumps({"copy_probe_bytes": nbytes, "rc": rc}))
 
 
+def progress(**kw) -> None:
+    """One progress line on stderr (the JSON result line is on stdout)."""
+    print(json.dumps({"progress": kw}), file=sys.stderr, flush=True)
+
+
+def hc_node_count(model) -> int:
+    """Distinct nodes of a model's honeycomb elements."""
+    v = model.view
+    parts = np.ctypeslib.as_array(v.hex_part, shape=(v.n_hexes,))
+    kinds = np.array([v.materials[p].model for p in range(v.n_parts)], dtype=np.int32)
+    hexes = np.ctypeslib.as_array(v.hexes, shape=(v.n_hexes, 8))
+    return int(np.unique(hexes[kinds[parts] == 1]).size)
+
+
+def timed_runs(batch, steps: int, warmup: int, budget_s: float, label: str):
+    """At most one untimed warm-up run, then whole runs of the batch, each
+    bracketed by device synchronisation inside crush_batch_run, until
+    `steps` runs are timed or the next one would pass `budget_s` of timed
+    wall time (at least one is timed).  Returns the per-run stats."""
+    for w in range(min(warmup, 1)):
+        t0 = time.perf_counter()
+        batch.run()
+        batch.reset()
+        progress(leg=label, warmup=w + 1, s=round(time.perf_counter() - t0, 2))
+    out = []
+    while len(out) < max(steps, 1):
+        if out:
+            batch.reset()
+        st = batch.run()
+        out.append(st)
+        spent = sum(x.wall_ms for x in out) / 1e3
+        progress(leg=label, timed=len(out), wall_s=round(st.wall_ms / 1e3, 3), timed_s=round(spent, 2))
+        if spent + spent / len(out) > budget_s:
+            break
+    return out
+
+
+def force_roofline(tk, layout_bytes_per_hc_step: float) -> dict:
+    """Roofline of the honeycomb force kernel from a kernel-timing run: the
+    kernel's summed duration (HIP events around every launch, on the
+    kernels' stream) over its launches, against the bytes this layout must
+    move; the reference storage model's 776 B and the counter-measured
+    traffic beside it."""
+    k_launches = max(1, tk.launches)
+    avg_ms = tk.k_force_hc_ms / k_launches
+    hc_per_launch = tk.hc_element_steps / k_launches
+    alg = layout_bytes_per_hc_step * hc_per_launch
+    achieved = alg / (avg_ms * 1e-3) / 1e9
+    traffic = (TRAFFIC_HC_READ + TRAFFIC_HC_WRITE) * hc_per_launch
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": "k_force_hc", "launch_ms": avg_ms, "launches": int(tk.launches),
+            "element_steps_per_launch": hc_per_launch,
+            "algorithmic_bytes_per_launch": alg, "bytes_per_element_step": layout_bytes_per_hc_step,
+            "algorithmic_definition": (f"{LAYOUT_HC_ELEM} B per honeycomb element-step (connectivity 32, class 4, "
+                                       "erosion 1, state 100 read + 100 written, corner forces 96) + 24 B per "
+                                       "distinct node of the honeycomb elements (displacement), this layout's "
+                                       "minimum; geometry constants are per class and stay in cache"),
+            "traffic_gbs": traffic / (avg_ms * 1e-3) / 1e9,
+            "reference_storage_gbs": B_HC * hc_per_launch / (avg_ms * 1e-3) / 1e9,
+            "reference_storage_note": ("776 B per element-step, the reference's per-element storage (geometry "
+                                       "constants 540 B per element); not moved by this layout")}
+
+
 def bench_sweep(args):
     """The batched workloads: the head-on/offset sweep and the barrier
-    calibration set, solved together by the batched path."""
+    calibration set, solved together by the batched path.  One bench step
+    is one run of the whole batch to the end of every instance."""
     t0 = time.perf_counter()
     if args.workload == "calib":
         cases = workloads.calibration_set(args.instances, seed=args.seed)
@@ -213,61 +333,55 @@ def bench_sweep(args):
     models = workloads.build_cases(cases)
     t_build = time.perf_counter() - t0
     n_hc = np.array([workloads.honeycomb_count(m) for m in models])
+    n_hc_nodes = np.array([hc_node_count(m) for m in models])
     E = np.array([m.n_hexes for m in models])
     N = np.array([m.n_nodes for m in models])
+    progress(leg="build", models=len(models), s=round(t_build, 2))
     opt = ca.batch_options(threads_per_instance=args.threads_per_instance, force_tile=args.force_tile,
                            steps_per_graph=args.steps_per_graph, autotune=int(args.autotune))
     batch = ca.Batch(models, opt)
     if args.eager:
         batch.time_kernels(True)
-    for _ in range(args.warmup):
-        batch.run()
-        batch.reset()
-    dev_ms, wall, lat50, lat99, launches = [], [], [], [], 0
-    for k in range(args.steps):
-        if k:
-            batch.reset()
-        st = batch.run()
-        dev_ms.append(st.device_ms)
-        wall.append(st.wall_ms)
-        lat50.append(st.latency_p50_ms)
-        lat99.append(st.latency_p99_ms)
-        launches = st.launches
+    runs = timed_runs(batch, args.steps, args.warmup, args.budget_s, "batch")
+    st = runs[-1]
+    n_timed = len(runs)
     steps = np.array([batch.result(i, fields=False).steps for i in range(len(models))])
     n = len(models)
     es = float((E * steps).sum())
-    t_run = sum(wall) / 1e3
-    value = n * args.steps / t_run
+    t_run = sum(r.wall_ms for r in runs) / 1e3
+    value = n * n_timed / t_run
     total_bytes = sum(step_bytes(int(n_hc[i]), int(E[i] - n_hc[i]), int(N[i]), int(steps[i])) for i in range(n))
+    # per honeycomb element-step, this layout's bytes (element part plus the
+    # honeycomb nodes' displacement, weighted by each instance's steps)
+    hc_es = float((n_hc * steps).sum())
+    layout = LAYOUT_HC_ELEM + LAYOUT_HC_NODE * float((n_hc_nodes * steps).sum()) / max(hc_es, 1.0)
     # roofline of the dominant kernel, the honeycomb element-force tiles:
-    # a second solve of the same sweep with HIP events recorded around every
+    # one more run of the same batch with HIP events recorded around every
     # launch (on the stream the kernels run on) gives the summed duration
     if args.eager:
         tk = st
     else:
         batch.time_kernels(True)
         batch.reset()
+        s0 = time.perf_counter()
         tk = batch.run()
         batch.time_kernels(False)
-    k_launches = max(1, tk.launches)
-    avg_ms = tk.k_force_hc_ms / k_launches
-    bytes_per_launch = B_HC * tk.hc_element_steps / k_launches
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+        progress(leg="kernel_timing", s=round(time.perf_counter() - s0, 2))
     el_steps = tk.element_steps - tk.hc_element_steps
     kernels = {
         "ms": {name: tk.k_ms[j] for j, name in enumerate(KERNELS)},
         "launches_each": tk.launches,
-        "force_el_gbs": B_EL * el_steps / max(tk.k_force_el_ms, 1e-9) / 1e6,
-        "whole_step_algorithmic_gbs": total_bytes / (tk.device_ms * 1e-3) / 1e9,
+        "force_el_gbs_reference_storage": B_EL * el_steps / max(tk.k_force_el_ms, 1e-9) / 1e6,
+        "whole_step_reference_storage_gbs": total_bytes / (tk.device_ms * 1e-3) / 1e9,
     }
     out = {
         "metric": METRIC,
         "value": value,
         "unit": "solves/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * t_run / args.steps,
+        "steps": n_timed,
+        "warmup": min(args.warmup, 1),
+        "ms_per_step": 1e3 * t_run / n_timed,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -281,32 +395,37 @@ def bench_sweep(args):
             "threads_per_instance": batch.options.threads_per_instance, "force_tile": batch.options.force_tile,
             "contact_cell_scale": batch.options.contact_cell_scale, "steps_per_graph": batch.options.steps_per_graph,
         },
-        "gelem_steps_per_s": es * args.steps / t_run / 1e9,
-        "latency_ms": {"p50": float(np.mean(lat50)), "p99": float(np.mean(lat99)),
+        "step_definition": (f"one run of all {n} instances to their end; steps requested {args.steps}, warm-up "
+                            f"requested {args.warmup}; runs are timed until {args.steps} are done or the next would "
+                            f"pass {args.budget_s:.0f} s of timed wall time, warm-up capped at one run"),
+        "gelem_steps_per_s": es * n_timed / t_run / 1e9,
+        "latency_ms": {"p50": float(np.mean([r.latency_p50_ms for r in runs])),
+                       "p99": float(np.mean([r.latency_p99_ms for r in runs])),
                        "definition": "batch start to instance retirement"},
-        "device_ms": float(np.mean(dev_ms)),
+        "device_ms": float(np.mean([r.device_ms for r in runs])),
         "host": {"build_models_s": t_build, "prepare_ms": st.prepare_ms, "upload_ms": st.upload_ms,
                  "pool_gib": st.pool_bytes / 2**30},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (TRAFFIC_HC_READ + TRAFFIC_HC_WRITE) * tk.hc_element_steps / k_launches,
-                     "traffic_gbs": (TRAFFIC_HC_READ + TRAFFIC_HC_WRITE) * tk.hc_element_steps / k_launches
-                     / (avg_ms * 1e-3) / 1e9,
-                     "kernel": "k_force_hc", "launch_ms": avg_ms,
-                     "algorithmic_bytes_per_launch": bytes_per_launch, "bytes_per_element_step": B_HC},
+        "roofline": force_roofline(tk, layout),
         "kernels": kernels,
     }
     if not args.no_plain:
+        s0 = time.perf_counter()
         pg = plain_gpu(models)
         out["plain_gpu"] = pg
         out["speedup_vs_plain"] = value / pg["value"]
+        progress(leg="plain_gpu", s=round(time.perf_counter() - s0, 2))
     if not args.no_cpu:
+        s0 = time.perf_counter()
         cb = cpu_baseline(models, es / n, budget_s=15.0)
         out["cpu_baseline"] = cb
         out["speedup_vs_cpu"] = value / cb["value"]
+        out["speedup_vs_cpu_all_cores_extrapolated"] = value / cb["all_cores_extrapolated"]
+        progress(leg="cpu_lanes", s=round(time.perf_counter() - s0, 2))
+        s0 = time.perf_counter()
         f64 = cpu_baseline(models, es / n, budget_s=8.0, path="f64")
         out["cpu_f64_oracle"] = f64
         out["speedup_vs_cpu_f64"] = value / f64["value"]
+        progress(leg="cpu_f64", s=round(time.perf_counter() - s0, 2))
     else:
         out["cpu_baseline"] = None
     return out
@@ -432,35 +551,26 @@ def bench_large(args):
     opt = ca.batch_options(threads_per_instance=args.threads_per_instance, force_tile=args.force_tile,
                            steps_per_graph=args.steps_per_graph, autotune=int(args.autotune))
     batch = ca.Batch([m], opt)
-    for _ in range(args.warmup):
-        batch.run()
-        batch.reset()
-    wall = []
-    for k in range(args.steps):
-        if k:
-            batch.reset()
-        wall.append(batch.run().wall_ms)
+    runs = timed_runs(batch, args.steps, args.warmup, args.budget_s, "batch")
+    n_timed = len(runs)
     r = batch.result(0, fields=False)
     steps = r.steps
-    t_run = sum(wall) / 1e3
-    value = args.steps / t_run
+    t_run = sum(x.wall_ms for x in runs) / 1e3
+    value = n_timed / t_run
     batch.time_kernels(True)
     batch.reset()
     tk = batch.run()
     batch.time_kernels(False)
-    k_launches = max(1, tk.launches)
-    avg_ms = tk.k_force_hc_ms / k_launches
-    bytes_per_launch = B_HC * tk.hc_element_steps / k_launches
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
+    layout = LAYOUT_HC_ELEM + LAYOUT_HC_NODE * hc_node_count(m) / max(n_hc, 1)
     es = float(m.n_hexes * steps)
     out = {
         "metric": METRIC,
         "value": value,
         "unit": "solves/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * t_run / args.steps,
+        "steps": n_timed,
+        "warmup": min(args.warmup, 1),
+        "ms_per_step": 1e3 * t_run / n_timed,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -471,16 +581,11 @@ def bench_large(args):
                                 f"{args.end_time} s"),
                    "elements": m.n_hexes, "nodes": m.n_nodes, "honeycomb": n_hc, "solver_steps": int(steps),
                    "force_tile": batch.options.force_tile, "steps_per_graph": batch.options.steps_per_graph},
-        "gelem_steps_per_s": es * args.steps / t_run / 1e9,
-        "us_per_solver_step": 1e6 * t_run / args.steps / max(steps, 1),
+        "gelem_steps_per_s": es * n_timed / t_run / 1e9,
+        "us_per_solver_step": 1e6 * t_run / n_timed / max(steps, 1),
         "delta_v": [g.delta_v[0] for g in r.groups],
         "host": {"build_model_s": t_build},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": (TRAFFIC_HC_READ + TRAFFIC_HC_WRITE) * tk.hc_element_steps / k_launches,
-                     "kernel": "k_force_hc",
-                     "launch_ms": avg_ms, "algorithmic_bytes_per_launch": bytes_per_launch,
-                     "bytes_per_element_step": B_HC},
+        "roofline": force_roofline(tk, layout),
         "kernels": {"ms": {name: tk.k_ms[j] for j, name in enumerate(KERNELS)}, "launches_each": tk.launches},
     }
     if not args.no_plain:
@@ -494,7 +599,8 @@ def bench_large(args):
         sys.path.insert(0, str(ROOT / "tests"))
         import oracle_ffi as of
         lib = of.load()
-        cores = max(1, min(os.cpu_count() or 1, 16))
+        host = host_cpus()
+        cores = host["threads"]
         done = C.c_int32()
         n_s = 40
         s0 = time.perf_counter()
@@ -507,9 +613,10 @@ def bench_large(args):
             "value": 1.0 / (per_step * steps), "unit": "solves/s", "cores": cores, "kind": "port",
             "sample": (f"AVX2 8-lane f32 restatement of the reference's default element path, {done.value} steps of "
                        f"this model with {cores} OpenMP threads in the element pass (the rest of the step "
-                       f"sequential, as in the reference) on {cpu_info()}; {per_step * 1e3:.1f} ms per step, set-up "
-                       f"({s1 - s0:.1f} s) excluded, scaled to the GPU's {steps} steps"),
+                       f"sequential, as in the reference) on {host['cpu_model']}; {per_step * 1e3:.1f} ms per step, "
+                       f"set-up ({s1 - s0:.1f} s) excluded, scaled to the GPU's {steps} steps"),
             "gelem_steps_per_s": m.n_hexes / per_step / 1e9,
+            "host": host,
         }
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
     else:
@@ -654,6 +761,8 @@ def main():
     ap.add_argument("--force-tile", type=int, default=256)
     ap.add_argument("--steps-per-graph", type=int, default=None)
     ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--budget-s", type=float, default=90.0,
+                    help="batched workloads: timed runs stop once the next would pass this much timed wall time")
     ap.add_argument("--end-time", type=float, default=0.15,
                     help="simulated time per instance (0.15 s; shorter only for profiling runs)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
