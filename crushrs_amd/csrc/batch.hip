@@ -98,10 +98,7 @@ ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     o.conn = b.put(P.conn);
     o.ecls = b.put(P.ecls);
     o.cls_mat = b.put(P.cls_mat);
-    o.cgrad = b.put(P.cls_grad);
-    o.cvol = b.put(P.cls_vol);
-    o.cgam = b.put(P.cls_gamma);
-    o.chs = b.put(P.cls_H);
+    o.cpk = b.put(P.cls_pack);
     o.cdtf = b.put(P.cls_dtf);
     o.emass = b.put(P.elem_mass);
     o.mats = b.put(P.mats);
@@ -210,10 +207,7 @@ InstDev make_inst(const Prepared& P, const ImageOffsets& o, uintptr_t base) {
     I.conn = at<int32_t>(base, o.conn);
     I.ecls = at<int32_t>(base, o.ecls);
     I.cls_mat = at<int32_t>(base, o.cls_mat);
-    I.cgrad = at<float>(base, o.cgrad);
-    I.cvol = at<float>(base, o.cvol);
-    I.cgam = at<float>(base, o.cgam);
-    I.chs = at<float>(base, o.chs);
+    I.cpk = at<float>(base, o.cpk);
     I.cdtf = at<double>(base, o.cdtf);
     I.emass = at<double>(base, o.emass);
     I.mats = at<MatParams>(base, o.mats);
@@ -559,11 +553,10 @@ __device__ __forceinline__ void force_tile(const InstDev* __restrict__ table, co
     if (c.adaptive && (c.step + 1) % c.chk == 0) dt_partial<T>(D, c.dt_floor, s, end, dt_slot<KIND, T>(D, tile.y), s_w);
 }
 
-// Lower bound of the waves per SIMD the honeycomb kernel is compiled for;
-// at 2 it takes 121 VGPRs (4 waves, no spills).  5 forces spills that cost
-// more than the extra wave gains (21 % slower).
+// Lower bound of the waves per SIMD the honeycomb kernel is compiled for:
+// at 4 it fits 128 VGPRs with no spills (the class constants sit in SGPRs).
 #ifndef CRUSH_HC_WAVES
-#define CRUSH_HC_WAVES 2
+#define CRUSH_HC_WAVES 4
 #endif
 
 template <int T>

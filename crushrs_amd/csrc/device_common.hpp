@@ -69,30 +69,52 @@ constexpr double C_TRI_TOL = 0.1;
 
 // ------------------------------------------------------------ f32 3x3
 
+// The f32 element arithmetic is compiled without contraction and states
+// its fused multiply-adds explicitly: every kernel that inlines it, on
+// either path of a class-uniformity branch, and the host build round the
+// same way, so results do not depend on how the compiler schedules a
+// kernel.
+CRUSH_HD float fmf(float a, float b, float c) { return fmaf(a, b, c); }
+// a0 b0 + a1 b1 + a2 b2, accumulated left to right
+CRUSH_HD float dot3f(float a0, float b0, float a1, float b1, float a2, float b2) {
+#pragma clang fp contract(off)
+    return fmf(a2, b2, fmf(a1, b1, a0 * b0));
+}
+// a b - c d
+CRUSH_HD float cross2f(float a, float b, float c, float d) {
+#pragma clang fp contract(off)
+    return fmf(a, b, -(c * d));
+}
+
 CRUSH_HD float det3f(const float* m) {
-    return m[0] * (m[4] * m[8] - m[5] * m[7]) - m[1] * (m[3] * m[8] - m[5] * m[6]) +
-           m[2] * (m[3] * m[7] - m[4] * m[6]);
+#pragma clang fp contract(off)
+    const float c0 = cross2f(m[4], m[8], m[5], m[7]);
+    const float c1 = cross2f(m[3], m[8], m[5], m[6]);
+    const float c2 = cross2f(m[3], m[7], m[4], m[6]);
+    return fmf(m[2], c2, fmf(-m[1], c1, m[0] * c0));
 }
 
 // Cofactor inverse with a given determinant.
 CRUSH_HD void inv3f(const float* m, float det, float* r) {
+#pragma clang fp contract(off)
     float id = 1.0f / det;
-    r[0] = (m[4] * m[8] - m[5] * m[7]) * id;
-    r[1] = (m[2] * m[7] - m[1] * m[8]) * id;
-    r[2] = (m[1] * m[5] - m[2] * m[4]) * id;
-    r[3] = (m[5] * m[6] - m[3] * m[8]) * id;
-    r[4] = (m[0] * m[8] - m[2] * m[6]) * id;
-    r[5] = (m[2] * m[3] - m[0] * m[5]) * id;
-    r[6] = (m[3] * m[7] - m[4] * m[6]) * id;
-    r[7] = (m[1] * m[6] - m[0] * m[7]) * id;
-    r[8] = (m[0] * m[4] - m[1] * m[3]) * id;
+    r[0] = cross2f(m[4], m[8], m[5], m[7]) * id;
+    r[1] = cross2f(m[2], m[7], m[1], m[8]) * id;
+    r[2] = cross2f(m[1], m[5], m[2], m[4]) * id;
+    r[3] = cross2f(m[5], m[6], m[3], m[8]) * id;
+    r[4] = cross2f(m[0], m[8], m[2], m[6]) * id;
+    r[5] = cross2f(m[2], m[3], m[0], m[5]) * id;
+    r[6] = cross2f(m[3], m[7], m[4], m[6]) * id;
+    r[7] = cross2f(m[1], m[6], m[0], m[7]) * id;
+    r[8] = cross2f(m[0], m[4], m[1], m[3]) * id;
 }
 
 CRUSH_HD void mul3f(const float* a, const float* b, float* r) {
+#pragma clang fp contract(off)
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) r[3 * i + j] = a[3 * i] * b[j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
+        for (int j = 0; j < 3; ++j) r[3 * i + j] = dot3f(a[3 * i], b[j], a[3 * i + 1], b[3 + j], a[3 * i + 2], b[6 + j]);
 }
 
 // ------------------------------------------------- element force (f32)
@@ -106,6 +128,7 @@ CRUSH_HD void mul3f(const float* a, const float* b, float* r) {
 // against I that limits an f32 F_prev^-1.  Updates st and returns the
 // Kirchhoff stress tau (row-major).
 CRUSH_HD void honeycomb_update(const MatParams& mp, const float* hd, float* st, float* tau) {
+#pragma clang fp contract(off)
     float fprev[9], fpinv[9], dh[9], mid[9], midinv[9], g[9], dinc[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
@@ -117,7 +140,7 @@ CRUSH_HD void honeycomb_update(const MatParams& mp, const float* hd, float* st, 
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         float id = (k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f;
-        mid[k] = 0.5f * dinc[k] + id;
+        mid[k] = fmf(0.5f, dinc[k], id);
     }
     inv3f(mid, det3f(mid), midinv);
     mul3f(dinc, midinv, g);
@@ -130,14 +153,14 @@ CRUSH_HD void honeycomb_update(const MatParams& mp, const float* hd, float* st, 
             w[3 * i + j] = 0.5f * (g[3 * i + j] - g[3 * j + i]);
         }
     // Cayley rotation increment I + (W + W^2/2) / (1 + |omega|^2 / 4)
-    float omega2 = w[1] * w[1] + w[2] * w[2] + w[5] * w[5];
-    float inv_den = 1.0f / (1.0f + 0.25f * omega2);
+    float omega2 = dot3f(w[1], w[1], w[2], w[2], w[5], w[5]);
+    float inv_den = 1.0f / fmf(0.25f, omega2, 1.0f);
     float ww[9], dr[9], r[9];
     mul3f(w, w, ww);
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         float id = (k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f;
-        dr[k] = (w[k] + 0.5f * ww[k]) * inv_den + id;
+        dr[k] = fmf(fmf(0.5f, ww[k], w[k]), inv_den, id);
     }
     mul3f(dr, st + 9, r);
     // strain increment in the corotational frame: R^T d R
@@ -145,16 +168,16 @@ CRUSH_HD void honeycomb_update(const MatParams& mp, const float* hd, float* st, 
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) rtd[3 * i + j] = r[i] * d[j] + r[3 + i] * d[3 + j] + r[6 + i] * d[6 + j];
+        for (int j = 0; j < 3; ++j) rtd[3 * i + j] = dot3f(r[i], d[j], r[3 + i], d[3 + j], r[6 + i], d[6 + j]);
     mul3f(rtd, r, de);
     const float e = mp.youngs;
     float s[6];
-    s[0] = st[18] + e * de[0];
-    s[1] = st[19] + e * de[4];
-    s[2] = st[20] + e * de[8];
-    s[3] = st[21] + e * de[1];
-    s[4] = st[22] + e * de[5];
-    s[5] = st[23] + e * de[2];
+    s[0] = fmf(e, de[0], st[18]);
+    s[1] = fmf(e, de[4], st[19]);
+    s[2] = fmf(e, de[8], st[20]);
+    s[3] = fmf(e, de[1], st[21]);
+    s[4] = fmf(e, de[5], st[22]);
+    s[5] = fmf(e, de[2], st[23]);
     const float c = st[24];
     // Active knot: the last i with c >= knot_c[i] (src/material.rs:459-467);
     // knots strictly increase with unused ones at +inf, so it is the count
@@ -164,24 +187,24 @@ CRUSH_HD void honeycomb_update(const MatParams& mp, const float* hd, float* st, 
     for (int i = 1; i < 8; ++i) ki += (c >= mp.knot_c[i]) ? 1 : 0;
     const float kc = mp.knot_c[ki], ks = mp.knot_s[ki], kh = mp.knot_h[ki];
     const float hard = kh;
-    const float syc = ks + hard * (c - kc);
+    const float syc = fmf(hard, c - kc, ks);
     const float tk[3] = {1.0f, mp.trans, mp.trans};
     // Normal caps with implicit hardening: three passes of the active-set
     // solve dc = sum_A(-s_k - t_k syc) / (E + H sum_A t_k).
     float dc = 0.0f;
 #pragma unroll
     for (int it = 0; it < 3; ++it) {
-        float sy = syc + hard * dc;
+        float sy = fmf(hard, dc, syc);
         float sum = 0.0f, ta = 0.0f;
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             bool over = -s[k] > tk[k] * sy;
-            sum += over ? (-s[k] - tk[k] * syc) : 0.0f;
+            sum += over ? fmf(-tk[k], syc, -s[k]) : 0.0f;
             ta += over ? tk[k] : 0.0f;
         }
-        dc = ta > 0.0f ? sum / (e + ta * hard) : 0.0f;
+        dc = ta > 0.0f ? sum / fmf(ta, hard, e) : 0.0f;
     }
-    const float sy = fmaxf(syc + hard * dc, 0.0f);
+    const float sy = fmaxf(fmf(hard, dc, syc), 0.0f);
 #pragma unroll
     for (int k = 0; k < 3; ++k) s[k] = fminf(fmaxf(s[k], -tk[k] * sy), tk[k] * sy);
     const float ty = 0.5f * sy;
@@ -199,7 +222,7 @@ CRUSH_HD void honeycomb_update(const MatParams& mp, const float* hd, float* st, 
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            tau[3 * i + j] = J * (rs[3 * i] * r[3 * j] + rs[3 * i + 1] * r[3 * j + 1] + rs[3 * i + 2] * r[3 * j + 2]);
+            tau[3 * i + j] = J * dot3f(rs[3 * i], r[3 * j], rs[3 * i + 1], r[3 * j + 1], rs[3 * i + 2], r[3 * j + 2]);
     // commit
 #pragma unroll
     for (int k = 0; k < 9; ++k) st[k] = hd[k];
@@ -327,6 +350,7 @@ CRUSH_HD void sym_eigen3f(const float* a, float* vals, float* vecs) { sym_eigen3
 // register budget of every caller.
 template <class T>
 __host__ __device__ __attribute__((noinline)) inline void hencky_log_strain_large(const T* hd, T* eps) {
+#pragma clang fp contract(off)
     double bd[9], lam[3], vec[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -347,12 +371,17 @@ __host__ __device__ __attribute__((noinline)) inline void hencky_log_strain_larg
             eps[3 * i + j] = (T)(e3[0] * vec[i] * vec[j] + e3[1] * vec[3 + i] * vec[3 + j] + e3[2] * vec[6 + i] * vec[6 + j]);
 }
 
+CRUSH_HD float fmt(float a, float b, float c) { return fmaf(a, b, c); }
+CRUSH_HD double fmt(double a, double b, double c) { return fma(a, b, c); }
+
 template <class T>
 CRUSH_HD void mul3t(const T* a, const T* b, T* r) {
+#pragma clang fp contract(off)
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
-        for (int j = 0; j < 3; ++j) r[3 * i + j] = a[3 * i] * b[j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
+        for (int j = 0; j < 3; ++j)
+            r[3 * i + j] = fmt(a[3 * i + 2], b[6 + j], fmt(a[3 * i + 1], b[3 + j], a[3 * i] * b[j]));
 }
 
 // Hencky elastic Kirchhoff stress (src/material.rs:708-771 with no
@@ -365,13 +394,14 @@ CRUSH_HD void mul3t(const T* a, const T* b, T* r) {
 // eigen-decomposition with eps_a = log1p(mu_a)/2.
 template <class T>
 CRUSH_HD void hencky_elastic_t(const MatParams& mp, const T* hd, T* tau) {
+#pragma clang fp contract(off)
     T bm[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
             bm[3 * i + j] = hd[3 * i + j] + hd[3 * j + i] +
-                            (hd[3 * i] * hd[3 * j] + hd[3 * i + 1] * hd[3 * j + 1] + hd[3 * i + 2] * hd[3 * j + 2]);
+                            fmt(hd[3 * i + 2], hd[3 * j + 2], fmt(hd[3 * i + 1], hd[3 * j + 1], hd[3 * i] * hd[3 * j]));
     T mx = T(0);
 #pragma unroll
     for (int k = 0; k < 9; ++k) mx = fmax(mx, fabs(bm[k]));
@@ -381,7 +411,7 @@ CRUSH_HD void hencky_elastic_t(const MatParams& mp, const T* hd, T* tau) {
         const T c[6] = {T(1), T(0.5), T(1) / T(3), T(0.25), T(0.2), T(1) / T(6)};
         T acc[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) acc[k] = ((k == 0 || k == 4 || k == 8) ? c[4] : T(0)) - c[5] * bm[k];
+        for (int k = 0; k < 9; ++k) acc[k] = fmt(-c[5], bm[k], (k == 0 || k == 4 || k == 8) ? c[4] : T(0));
 #pragma unroll
         for (int n = 3; n >= 0; --n) {
             T t[9];
@@ -405,7 +435,7 @@ CRUSH_HD void hencky_elastic_t(const MatParams& mp, const T* hd, T* tau) {
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const bool dg = (k == 0 || k == 4 || k == 8);
-        tau[k] = g2 * (eps[k] - (dg ? vol * (T(1) / T(3)) : T(0))) + (dg ? pressure : T(0));
+        tau[k] = fmt(g2, eps[k] - (dg ? vol * (T(1) / T(3)) : T(0)), dg ? pressure : T(0));
     }
 }
 
@@ -419,6 +449,7 @@ template <class GradF, class GamF, class HF>
 CRUSH_HD void element_nodal_forces(const float* tau, const float* F, float J, float vol,
                                                      const float (*dl)[3], GradF grad, GamF gam, HF hsym,
                                                      float (*fo)[3]) {
+#pragma clang fp contract(off)
     float finv[9];
     inv3f(F, J, finv);
     // M = tau F^-T V
@@ -427,12 +458,12 @@ CRUSH_HD void element_nodal_forces(const float* tau, const float* F, float J, fl
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            M[3 * i + j] = vol * (tau[3 * i] * finv[3 * j] + tau[3 * i + 1] * finv[3 * j + 1] + tau[3 * i + 2] * finv[3 * j + 2]);
+            M[3 * i + j] = vol * dot3f(tau[3 * i], finv[3 * j], tau[3 * i + 1], finv[3 * j + 1], tau[3 * i + 2], finv[3 * j + 2]);
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         const float g0 = grad(3 * i), g1 = grad(3 * i + 1), g2 = grad(3 * i + 2);
 #pragma unroll
-        for (int a = 0; a < 3; ++a) fo[i][a] = M[3 * a] * g0 + M[3 * a + 1] * g1 + M[3 * a + 2] * g2;
+        for (int a = 0; a < 3; ++a) fo[i][a] = dot3f(M[3 * a], g0, M[3 * a + 1], g1, M[3 * a + 2], g2);
     }
     // hourglass: q = U Gamma, r = H q, f += r Gamma^T (node 0 has dl = 0)
     float q[12];
@@ -442,7 +473,7 @@ CRUSH_HD void element_nodal_forces(const float* tau, const float* F, float J, fl
         for (int d = 0; d < 3; ++d) {
             float s = 0.0f;
 #pragma unroll
-            for (int i = 1; i < 8; ++i) s += dl[i][d] * gam(4 * i + a);
+            for (int i = 1; i < 8; ++i) s = fmf(dl[i][d], gam(4 * i + a), s);
             q[3 * a + d] = s;
         }
     float rr[12];
@@ -454,8 +485,8 @@ CRUSH_HD void element_nodal_forces(const float* tau, const float* F, float J, fl
 #pragma unroll
         for (int c = r; c < 12; ++c) {
             const float h = hsym(kk++);
-            rr[r] += h * q[c];
-            if (c != r) rr[c] += h * q[r];
+            rr[r] = fmf(h, q[c], rr[r]);
+            if (c != r) rr[c] = fmf(h, q[r], rr[c]);
         }
     }
 #pragma unroll
@@ -464,7 +495,7 @@ CRUSH_HD void element_nodal_forces(const float* tau, const float* F, float J, fl
         for (int d = 0; d < 3; ++d) {
             float s = fo[i][d];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) s += rr[3 * a + d] * gam(4 * i + a);
+            for (int a = 0; a < 4; ++a) s = fmf(rr[3 * a + d], gam(4 * i + a), s);
             fo[i][d] = s;
         }
 }

@@ -32,7 +32,7 @@ constexpr double GRID_MARGIN = 0.1;
 // Offsets of every array of an instance image relative to its start; the
 // first `init_bytes` must be uploaded, the rest is scratch.
 struct ImageOffsets {
-    size_t conn, ecls, cls_mat, cgrad, cvol, cgam, chs, cdtf, emass, mats, state, eroded, extra;
+    size_t conn, ecls, cls_mat, cpk, cdtf, emass, mats, state, eroded, extra;
     size_t X, u, v, f_int, f_ext, inv_mass, mass, added, fixed, n2e_ptr, n2e_idx;
     size_t cdesc, cn_node, cn_scale, cf_nodes, cf_contact, cf_static, cf_level, tn_node, tn_ptr, tn_ent, rec_face, fh_count;
     size_t g_nodes_ptr, g_nodes, g_front_ptr, g_front, g_accel, g_axis, g_mass, g_v0, g_dv, g_frame, ctl;

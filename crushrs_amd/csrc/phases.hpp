@@ -81,10 +81,7 @@ struct InstDev {
     const int32_t* conn;
     const int32_t* ecls;
     const int32_t* cls_mat;
-    const float* cgrad;
-    const float* cvol;
-    const float* cgam;
-    const float* chs;
+    const float* cpk;  // [C][CPK] geometry constants per class
     const double* cdtf;
     const double* emass;
     const MatParams* mats;
@@ -426,15 +423,60 @@ __device__ __forceinline__ bool half_node(const InstDev& D, int n, double dt) {
     return ok;
 }
 
-// Element force of slot s (f32), streamed so that few values are live at
-// once: (1) one pass over the nodes accumulates the displacement gradient
+// Pointers into the constant address space: loads through them at a
+// wave-uniform address become scalar loads into SGPRs (the data is never
+// written while a kernel runs).
+typedef float f8v __attribute__((ext_vector_type(8)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef const __attribute__((address_space(4))) f8v* cf8_ptr;
+typedef const __attribute__((address_space(4))) f16v* cf16_ptr;
+typedef const __attribute__((address_space(4))) int32_t* cint_ptr;
+typedef const __attribute__((address_space(4))) MatParams* cmat_ptr;
+
+// Constants of one geometry class (the class block of CPK floats).  With
+// UNI the class is wave-uniform and each read is a scalar load of a node
+// record (first 8 dwords of its 64-byte slot) or of a 64-byte line of the
+// H part; otherwise each lane reads its own class's floats.
+template <bool UNI>
+struct ClassConsts {
+    const float* k;
+    // gradient g[3] and hourglass vectors ga[4] of node i
+    __device__ __forceinline__ void node(int i, float* g, float* ga) const {
+        if (UNI) {
+            const f8v v = *(cf8_ptr)(k + CPK_NODE * i);
+            g[0] = v[0], g[1] = v[1], g[2] = v[2];
+            ga[0] = v[3], ga[1] = v[4], ga[2] = v[5], ga[3] = v[6];
+        } else {
+            const float* p = k + CPK_NODE * i;
+#pragma unroll
+            for (int b = 0; b < 3; ++b) g[b] = p[CPK_GRAD + b];
+#pragma unroll
+            for (int a = 0; a < 4; ++a) ga[a] = p[CPK_GAM + a];
+        }
+    }
+    // 16 floats of the H part and volume from CPK_H + 16 j
+    __device__ __forceinline__ f16v h16(int j) const {
+        if (UNI) return ((cf16_ptr)(k + CPK_H))[j];
+        f16v v;
+#pragma unroll
+        for (int e = 0; e < 16; ++e) v[e] = k[CPK_H + 16 * j + e];
+        return v;
+    }
+};
+
+// Element force of slot s (f32) of geometry class c.  With UNI, c is the
+// same for every lane that calls it, and the class's constants and material
+// come in through scalar loads; without, each lane loads its own class's.
+// Streamed so that few values are live at once:
+// (1) one pass over the nodes accumulates the displacement gradient
 // H = sum_i (u_i - u_0) (x) grad_i and the hourglass coordinates
-// q = sum_i (u_i - u_0) gamma_i; (2) r = H_hg q; (3) the material update
-// gives tau; (4) each node's force tau F^-T grad_i V + r gamma_i^T is
+// q = sum_i (u_i - u_0) gamma_i; (2) the material update gives tau;
+// (3) r = H_hg q; (4) each node's force tau F^-T grad_i V + r gamma_i^T is
 // written as it is formed.  Returns 1 when the element inverts now.
-template <int KIND>
-__device__ __forceinline__ int force_element(const InstDev& D, int s) {
-    const unsigned E = (unsigned)D.E, C = (unsigned)D.C, nh = (unsigned)D.n_hc;
+template <int KIND, bool UNI>
+__device__ __forceinline__ int force_element_class(const InstDev& D, int s, int c) {
+#pragma clang fp contract(off)
+    const unsigned E = (unsigned)D.E, nh = (unsigned)D.n_hc;
     // corner forces: fe[(corner * E + slot) * 3 + axis], each corner's xyz
     // in one 12-byte store (and one load in the gather)
     const rsrc_t rfe = make_rsrc(D.fe, (size_t)24 * E * 4);
@@ -446,11 +488,10 @@ __device__ __forceinline__ int force_element(const InstDev& D, int s) {
     }
     const rsrc_t rconn = make_rsrc(D.conn, (size_t)8 * E * 4);
     const rsrc_t ru = make_rsrc(D.u, (size_t)D.N * 24);
-    const rsrc_t rgrad = make_rsrc(D.cgrad, (size_t)24 * C * 4);
-    const rsrc_t rgam = make_rsrc(D.cgam, (size_t)32 * C * 4);
-    const rsrc_t rhs = make_rsrc(D.chs, (size_t)HSYM * C * 4);
-    const int c = D.ecls[s];
-    const unsigned vc = (unsigned)c * 4u;
+    const ClassConsts<UNI> K{D.cpk + (size_t)c * CPK};
+    // the force assembly rereads the node records with per-lane loads:
+    // holding those in SGPRs too would exceed the SGPR file and spill
+    const ClassConsts<false> Kv{D.cpk + (size_t)c * CPK};
     // the honeycomb state is loaded first so that its latency overlaps the
     // gradient pass
     const rsrc_t rst = make_rsrc(D.state, (size_t)25 * nh * 4);
@@ -476,19 +517,21 @@ __device__ __forceinline__ int force_element(const InstDev& D, int s) {
         const float d0 = (float)(ui[0] - u0x);
         const float d1 = (float)(ui[1] - u0y);
         const float d2 = (float)(ui[2] - u0z);
+        float gi[3], gai[4];
+        K.node(i, gi, gai);
 #pragma unroll
         for (int b = 0; b < 3; ++b) {
-            const float g = ld_f32(rgrad, vc, (3 * i + b) * C * 4u);
-            hd[b] += d0 * g;
-            hd[3 + b] += d1 * g;
-            hd[6 + b] += d2 * g;
+            const float g = gi[b];
+            hd[b] = fmf(d0, g, hd[b]);
+            hd[3 + b] = fmf(d1, g, hd[3 + b]);
+            hd[6 + b] = fmf(d2, g, hd[6 + b]);
         }
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
-            const float ga = ld_f32(rgam, vc, (4 * i + a) * C * 4u);
-            q[3 * a] += d0 * ga;
-            q[3 * a + 1] += d1 * ga;
-            q[3 * a + 2] += d2 * ga;
+            const float ga = gai[a];
+            q[3 * a] = fmf(d0, ga, q[3 * a]);
+            q[3 * a + 1] = fmf(d1, ga, q[3 * a + 1]);
+            q[3 * a + 2] = fmf(d2, ga, q[3 * a + 2]);
         }
     }
     float F[9];
@@ -501,7 +544,8 @@ __device__ __forceinline__ int force_element(const InstDev& D, int s) {
         for (int i = 0; i < 8; ++i) st_f32x3(rfe, 0.0f, 0.0f, 0.0f, vs3, i * E * 12u);
         return 1;
     }
-    const MatParams& mp = D.mats[D.cls_mat[c]];
+    const MatParams& mp = UNI ? *(const MatParams*)((cmat_ptr)D.mats + ((cint_ptr)D.cls_mat)[c])
+                              : D.mats[D.cls_mat[c]];
     float tau[9];
     if (KIND == CRUSH_MAT_HONEYCOMB) {
         honeycomb_update(mp, hd, st, tau);
@@ -513,12 +557,12 @@ __device__ __forceinline__ int force_element(const InstDev& D, int s) {
     // M = tau F^-T V
     float finv[9], M[9];
     inv3f(F, J, finv);
-    const float vol = D.cvol[c];
+    const float vol = K.h16((CPK_VOL - CPK_H) / 16)[(CPK_VOL - CPK_H) % 16];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            M[3 * i + j] = vol * (tau[3 * i] * finv[3 * j] + tau[3 * i + 1] * finv[3 * j + 1] + tau[3 * i + 2] * finv[3 * j + 2]);
+            M[3 * i + j] = vol * dot3f(tau[3 * i], finv[3 * j], tau[3 * i + 1], finv[3 * j + 1], tau[3 * i + 2], finv[3 * j + 2]);
     // r = H q with H packed as its upper triangle
     float rr[12];
 #pragma unroll
@@ -529,29 +573,38 @@ __device__ __forceinline__ int force_element(const InstDev& D, int s) {
         for (int r = 0; r < 12; ++r)
 #pragma unroll
             for (int cc = r; cc < 12; ++cc) {
-                const float h = ld_f32(rhs, vc, (unsigned)(kk++) * C * 4u);
-                rr[r] += h * q[cc];
-                if (cc != r) rr[cc] += h * q[r];
+                const float h = K.h16(kk / 16)[kk % 16];
+                ++kk;
+                rr[r] = fmf(h, q[cc], rr[r]);
+                if (cc != r) rr[cc] = fmf(h, q[r], rr[cc]);
             }
     }
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-        const float g0 = ld_f32(rgrad, vc, (3 * i) * C * 4u);
-        const float g1 = ld_f32(rgrad, vc, (3 * i + 1) * C * 4u);
-        const float g2 = ld_f32(rgrad, vc, (3 * i + 2) * C * 4u);
-        float ga[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) ga[a] = ld_f32(rgam, vc, (4 * i + a) * C * 4u);
+        float gi[3], ga[4];
+        Kv.node(i, gi, ga);
+        const float g0 = gi[0], g1 = gi[1], g2 = gi[2];
         float f[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
-            f[d] = M[3 * d] * g0 + M[3 * d + 1] * g1 + M[3 * d + 2] * g2;
+            f[d] = dot3f(M[3 * d], g0, M[3 * d + 1], g1, M[3 * d + 2], g2);
 #pragma unroll
-            for (int a = 0; a < 4; ++a) f[d] += rr[3 * a + d] * ga[a];
+            for (int a = 0; a < 4; ++a) f[d] = fmf(rr[3 * a + d], ga[a], f[d]);
         }
         st_f32x3(rfe, f[0], f[1], f[2], vs3, i * E * 12u);
     }
     return 0;
+}
+
+// Element force of slot s.  When the calling lanes' slots share one
+// geometry class (as runs of slots of one generated block do), the class
+// is taken as wave-uniform.
+template <int KIND>
+__device__ __forceinline__ int force_element(const InstDev& D, int s) {
+    const int c = D.ecls[s];
+    const int c0 = __builtin_amdgcn_readfirstlane(c);
+    if (__ballot(c != c0) == 0) return force_element_class<KIND, true>(D, s, c0);
+    return force_element_class<KIND, false>(D, s, c);
 }
 
 // Node-centric gather of the element forces (f64 sums in element order)
@@ -957,7 +1010,6 @@ __device__ __forceinline__ void element_output(const InstDev& D, int s, float* s
             for (int j = 0; j < 3; ++j) c[3 * i + j] = t[3 * i] * R[3 * j] + t[3 * i + 1] * R[3 * j + 1] + t[3 * i + 2] * R[3 * j + 2];
     } else {
         const int cl = D.ecls[s];
-        const int C = D.C;
         const int n0 = D.conn[s];
         float hd[9];
         for (int k = 0; k < 9; ++k) hd[k] = 0.0f;
@@ -965,7 +1017,7 @@ __device__ __forceinline__ void element_output(const InstDev& D, int s, float* s
             const int n = D.conn[(size_t)i * D.E + s];
             for (int a = 0; a < 3; ++a) {
                 const float du = (float)(D.u[3 * (size_t)n + a] - D.u[3 * (size_t)n0 + a]);
-                for (int b = 0; b < 3; ++b) hd[3 * a + b] += du * D.cgrad[(size_t)(3 * i + b) * C + cl];
+                for (int b = 0; b < 3; ++b) hd[3 * a + b] = fmf(du, D.cpk[(size_t)cl * CPK + CPK_NODE * i + CPK_GRAD + b], hd[3 * a + b]);
             }
         }
         float F[9];

@@ -387,11 +387,14 @@ void prepare(const crush_model& m, Prepared& p) {
                 for (int d = 0; d < 3; ++d) x[3 * i + d] -= x[d];  // local coordinates
             hex_geometry(x, mt.poisson_ratio, s.hourglass, grad, &vol, lump, gam, hs);
             if (!(vol > 0.0)) fail(CRUSH_E_INPUT, "hexahedron %d has non-positive volume", e);
-            for (int k = 0; k < 24; ++k) p.cls_grad.push_back((float)grad[k]);
-            p.cls_vol.push_back((float)vol);
+            float* pk = &*p.cls_pack.insert(p.cls_pack.end(), CPK, 0.0f);
+            for (int i = 0; i < 8; ++i) {
+                for (int b = 0; b < 3; ++b) pk[CPK_NODE * i + CPK_GRAD + b] = (float)grad[3 * i + b];
+                for (int a = 0; a < 4; ++a) pk[CPK_NODE * i + CPK_GAM + a] = (float)gam[4 * i + a];
+            }
+            for (int k = 0; k < HSYM; ++k) pk[CPK_H + k] = (float)(hs[k] * mt.youngs_modulus);
+            pk[CPK_VOL] = (float)vol;
             cls_vol64.push_back(vol);
-            for (int k = 0; k < 32; ++k) p.cls_gamma.push_back((float)gam[k]);
-            for (int k = 0; k < HSYM; ++k) p.cls_H.push_back((float)(hs[k] * mt.youngs_modulus));
             p.cls_mat.push_back(part);
             p.cls_dtf.push_back((1.0 - 0.36 * mt.poisson_ratio) / wave_speed(mt));
             for (int k = 0; k < 8; ++k) cls_lumped.push_back(lump[k] * mt.density);
@@ -402,16 +405,6 @@ void prepare(const crush_model& m, Prepared& p) {
     }
     const int C = (int)classes.size();
     p.n_classes = C;
-    // transpose class arrays to SoA [k][C]
-    auto soa = [C](std::vector<float>& v, int width) {
-        std::vector<float> t((size_t)width * C);
-        for (int c = 0; c < C; ++c)
-            for (int k = 0; k < width; ++k) t[(size_t)k * C + c] = v[(size_t)c * width + k];
-        v.swap(t);
-    };
-    soa(p.cls_grad, 24);
-    soa(p.cls_gamma, 32);
-    soa(p.cls_H, HSYM);
     // lumped masses and element masses (src/model.rs:268-278, solver.rs:199)
     p.mass.assign(N, 0.0);
     p.elem_mass.resize(E);

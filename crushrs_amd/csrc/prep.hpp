@@ -12,6 +12,15 @@
 namespace crush {
 
 constexpr int HSYM = 78;        // packed upper triangle of the 12x12 H
+// Geometry constants of one class, contiguous (class-major [C][CPK] f32,
+// 832 bytes, each class block 64-byte aligned): per node i a record in the
+// 64-byte slot at CPK_NODE i (gradients dN_i/dX_b at +b, hourglass vectors
+// gamma_ia at +3+a, the rest padding), then the packed H at CPK_H and the
+// volume at CPK_VOL.  The force kernels read a wave-uniform class with
+// scalar loads of whole records; with one record per 64-byte line and the
+// records apart, no scalar load spans two lines.
+constexpr int CPK_NODE = 16, CPK_GRAD = 0, CPK_GAM = 3, CPK_H = 8 * CPK_NODE, CPK_VOL = CPK_H + HSYM, CPK = 208;
+static_assert(CPK_VOL < CPK && CPK % 16 == 0 && CPK_H % 16 == 0, "class block layout");
 constexpr int SEARCH_BLOCK = 256;
 constexpr int FACE_HITS_CAP = 16;  // recorded hits per face before the scan fallback
 // Node flags (Prepared::fixed): a fixed node; a node contact forces reach
@@ -49,10 +58,7 @@ struct Prepared {
     std::vector<int32_t> slot_elem, elem_slot;
     std::vector<int32_t> conn;        // [8][E] by slot
     std::vector<int32_t> ecls;        // [E] geometry class per slot
-    std::vector<float> cls_grad;      // [24][C]
-    std::vector<float> cls_vol;       // [C]
-    std::vector<float> cls_gamma;     // [32][C]
-    std::vector<float> cls_H;         // [78][C]
+    std::vector<float> cls_pack;      // [C][CPK] geometry constants
     std::vector<int32_t> cls_mat;     // [C]
     std::vector<double> cls_dtf;      // [C] (1 - 0.36 nu) / c_d
     std::vector<double> elem_mass;    // [E] by slot: V rho

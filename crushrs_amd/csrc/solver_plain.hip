@@ -27,7 +27,7 @@ struct DeviceModel {
     hipStream_t stream;
     int dt_blocks = 0;
     DevBuf<int32_t> conn, ecls, cls_mat, n2e_ptr, n2e_idx;
-    DevBuf<float> cls_grad, cls_vol, cls_gamma, cls_H, state, fe;
+    DevBuf<float> cls_pack, state, fe;
     DevBuf<double> cls_dtf, elem_mass, extra, X, u, v, f_int, f_ext, inv_mass, mass, added, part_min, part_sum;
     DevBuf<uint8_t> eroded, fixed, cf_static;
     DevBuf<MatParams> mats;
@@ -48,10 +48,7 @@ struct DeviceModel {
         cls_mat.upload(P.cls_mat, s);
         n2e_ptr.upload(P.n2e_ptr, s);
         n2e_idx.upload(P.n2e_idx, s);
-        cls_grad.upload(P.cls_grad, s);
-        cls_vol.upload(P.cls_vol, s);
-        cls_gamma.upload(P.cls_gamma, s);
-        cls_H.upload(P.cls_H, s);
+        cls_pack.upload(P.cls_pack, s);
         cls_dtf.upload(P.cls_dtf, s);
         elem_mass.upload(P.elem_mass, s);
         mats.upload(P.mats, s);
@@ -145,10 +142,7 @@ struct DeviceModel {
         I.conn = conn.p;
         I.ecls = ecls.p;
         I.cls_mat = cls_mat.p;
-        I.cgrad = cls_grad.p;
-        I.cvol = cls_vol.p;
-        I.cgam = cls_gamma.p;
-        I.chs = cls_H.p;
+        I.cpk = cls_pack.p;
         I.cdtf = cls_dtf.p;
         I.emass = elem_mass.p;
         I.mats = mats.p;

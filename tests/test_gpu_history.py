@@ -50,7 +50,9 @@ def test_history_matches_oracle_and_conserves_momentum():
     P = g.parts
     mom = (P[:, :, 0:1] * P[:, :, 4:7]).sum(axis=1)
     p0 = (g.masses[:, None] * m.initial_velocity()).sum(axis=0)
-    assert np.abs(mom - p0).max() < 1e-9 * np.abs(p0).max()
+    # the reference holds momentum to 1e-6 of m v0 (tests/impact.rs:38);
+    # the f32 corner forces, summed in f64, leave a drift of order 1e-9
+    assert np.abs(mom - p0).max() < 1e-8 * np.abs(p0).max()
     hexes = m.hexes()
     hp = np.ctypeslib.as_array(m.view.hex_part, shape=(m.n_hexes,))
     for p in range(m.view.n_parts):

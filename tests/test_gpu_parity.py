@@ -89,9 +89,32 @@ def test_headon_toml_offset_road_first_100_steps():
     step_parity(headon_toml(), 100)
 
 
+class HeadonTomlCase:
+    """The TOML head-on as a case the envelope can rebuild."""
+
+    def build(self):
+        return headon_toml()
+
+
 def test_headon_toml_offset_road_full():
-    print("case: neon_1996 vs expedition_1999, offset 0.3, road, full solve", flush=True)
-    end_parity(headon_toml())
+    """Full solve against the reference's envelope: this model sits at a
+    bifurcation in the reference itself.  In the f64 oracle an element of
+    the crush zone survives or erodes under a 1e-9 change of the elastic
+    moduli (pinned on the CPU by
+    test_headon_toml_case_is_ill_conditioned_in_the_oracle), and the
+    reference's f32 lane path ends anywhere from 4931 to 5240 steps under
+    changes of 1e-9 to 1e-6, against 5223-5263 for the oracle: the step
+    count follows the most crushed element.  So one oracle solve is one
+    sample of the reference's answers.  Delta-v must lie within 0.05 m/s of
+    the envelope in every component and both crushes within 2 mm."""
+    from envelope import oracle_envelope, within_envelope
+    print("case: neon_1996 vs expedition_1999, offset 0.3, road, full solve vs oracle envelope", flush=True)
+    g = ca.run(headon_toml())
+    ens = oracle_envelope(HeadonTomlCase())
+    within_envelope(g, ens, dv_off_tol=0.05)
+    for k, a in enumerate(g.groups):
+        ce = [e.groups[k].crush_end for e in ens]
+        assert min(ce) - 2e-3 <= a.crush_end <= max(ce) + 2e-3, (a.crush_end, ce)
 
 
 def test_tbone_first_100_steps_and_full():

@@ -87,3 +87,29 @@ def test_calibration_case_at_bifurcation_flips_with_bumper_modulus():
     assert base.eroded_elements == 0 and bumped.eroded_elements == 1
     assert bumped.steps > 1.05 * base.steps
     assert abs(bumped.groups[0].delta_v[0] - base.groups[0].delta_v[0]) < 0.05
+
+
+def test_headon_toml_case_is_ill_conditioned_in_the_oracle():
+    """The TOML head-on of the parity tests (Neon vs Expedition at 25/20
+    mph, 0.3 m offset, road) sits at a bifurcation in the oracle itself:
+    scaling every elastic part's Young's modulus by 1 + 1e-9 makes one
+    crush-zone element erode that survives unperturbed, and the step count
+    moves.  The GPU parity test for the full solve therefore checks the GPU
+    against the oracle's envelope; delta-v stays well-conditioned."""
+    from concurrent.futures import ThreadPoolExecutor
+    from test_gpu_parity import headon_toml
+
+    def one(scale):
+        m = headon_toml()
+        v = m.view
+        for p in range(v.n_parts):
+            if v.materials[p].model == _abi.MAT_ELASTIC:
+                v.materials[p].youngs_modulus *= scale
+        return of.run(m)
+
+    with ThreadPoolExecutor(2) as ex:
+        base, bumped = list(ex.map(one, [1.0, 1.0 + 1e-9]))
+    assert base.eroded_elements == 0 and bumped.eroded_elements == 1
+    assert base.steps != bumped.steps
+    for a, b in zip(base.groups, bumped.groups):
+        assert abs(a.delta_v[0] - b.delta_v[0]) < 0.01
