@@ -7,6 +7,7 @@ Objects go to crushrs_amd/_build/, the library to crushrs_amd/libcrushrs_amd.so.
 from __future__ import annotations
 
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -30,17 +31,36 @@ def _sources():
     return sorted(list(CSRC.glob("*.cpp")) + list(CSRC.glob("*.hip")))
 
 
+def _sgpr_spills(remarks: str) -> dict:
+    """SGPR spill counts per kernel from -Rpass-analysis=kernel-resource-usage
+    remarks."""
+    out, name = {}, None
+    for line in remarks.splitlines():
+        m = re.search(r"Function Name: (\S+)", line)
+        if m:
+            name = m.group(1)
+        m = re.search(r"SGPRs Spill: (\d+)", line)
+        if m and name:
+            out[name] = int(m.group(1))
+    return out
+
+
 def _compile(src: Path) -> Path:
     obj = BUILD / (src.name + ".o")
     cmd = [HIPCC, *COMMON]
     if src.suffix == ".hip":
-        cmd += ["-x", "hip", *DEVICE]
+        cmd += ["-x", "hip", *DEVICE, "-Rpass-analysis=kernel-resource-usage"]
     else:
         cmd += ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include"]
     cmd += ["-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"compile failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    # the element force kernels must not spill SGPRs to VGPR lanes: a build
+    # that did gave batched results that changed from run to run
+    bad = {k: n for k, n in _sgpr_spills(r.stderr).items() if "k_force" in k and n > 0}
+    if bad:
+        raise RuntimeError(f"{src.name}: element force kernels spill SGPRs: {bad}")
     return obj
 
 

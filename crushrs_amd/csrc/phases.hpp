@@ -489,9 +489,11 @@ __device__ __forceinline__ int force_element_class(const InstDev& D, int s, int 
     const rsrc_t rconn = make_rsrc(D.conn, (size_t)8 * E * 4);
     const rsrc_t ru = make_rsrc(D.u, (size_t)D.N * 24);
     const ClassConsts<UNI> K{D.cpk + (size_t)c * CPK};
-    // the force assembly rereads the node records with per-lane loads:
-    // holding those in SGPRs too would exceed the SGPR file and spill
+    // the force assembly rereads the node records with per-lane loads, and
+    // the elastic kernel (whose Hencky law holds more SGPRs) reads H so
+    // too: holding those in SGPRs as well would exceed the SGPR file
     const ClassConsts<false> Kv{D.cpk + (size_t)c * CPK};
+    const ClassConsts<UNI && KIND == CRUSH_MAT_HONEYCOMB> Kh{D.cpk + (size_t)c * CPK};
     // the honeycomb state is loaded first so that its latency overlaps the
     // gradient pass
     const rsrc_t rst = make_rsrc(D.state, (size_t)25 * nh * 4);
@@ -557,7 +559,7 @@ __device__ __forceinline__ int force_element_class(const InstDev& D, int s, int 
     // M = tau F^-T V
     float finv[9], M[9];
     inv3f(F, J, finv);
-    const float vol = K.h16((CPK_VOL - CPK_H) / 16)[(CPK_VOL - CPK_H) % 16];
+    const float vol = Kh.h16((CPK_VOL - CPK_H) / 16)[(CPK_VOL - CPK_H) % 16];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -573,7 +575,7 @@ __device__ __forceinline__ int force_element_class(const InstDev& D, int s, int 
         for (int r = 0; r < 12; ++r)
 #pragma unroll
             for (int cc = r; cc < 12; ++cc) {
-                const float h = K.h16(kk / 16)[kk % 16];
+                const float h = Kh.h16(kk / 16)[kk % 16];
                 ++kk;
                 rr[r] = fmf(h, q[cc], rr[r]);
                 if (cc != r) rr[cc] = fmf(h, q[r], rr[cc]);
@@ -596,14 +598,19 @@ __device__ __forceinline__ int force_element_class(const InstDev& D, int s, int 
     return 0;
 }
 
-// Element force of slot s.  When the calling lanes' slots share one
-// geometry class (as runs of slots of one generated block do), the class
-// is taken as wave-uniform.
-template <int KIND>
+// Element force of slot s.  With SCALAR, when the calling lanes' slots
+// share one geometry class (as runs of slots of one generated block do),
+// the class is taken as wave-uniform and its constants come in through
+// scalar loads.  A kernel that inlines this must need no SGPR spills (the
+// build checks): a build whose force kernel spilled SGPRs to VGPR lanes
+// gave results that changed from run to run.
+template <int KIND, bool SCALAR = true>
 __device__ __forceinline__ int force_element(const InstDev& D, int s) {
     const int c = D.ecls[s];
-    const int c0 = __builtin_amdgcn_readfirstlane(c);
-    if (__ballot(c != c0) == 0) return force_element_class<KIND, true>(D, s, c0);
+    if (SCALAR) {
+        const int c0 = __builtin_amdgcn_readfirstlane(c);
+        if (__ballot(c != c0) == 0) return force_element_class<KIND, true>(D, s, c0);
+    }
     return force_element_class<KIND, false>(D, s, c);
 }
 

@@ -302,12 +302,13 @@ __global__ __launch_bounds__(256) void k_half(InstDev D) {
     half_node(D, n, D.ctl->dt);
 }
 
-// Element forces of slots [begin, end) (one material kind).
+// Element forces of slots [begin, end) (one material kind), with per-lane
+// loads of the class constants (the same arithmetic as the batched tiles).
 template <int KIND>
 __global__ __launch_bounds__(256) void k_force(InstDev D, int begin, int end) {
     const int s = begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= end || D.ctl->done) return;
-    if (force_element<KIND>(D, s)) atomicAdd(&D.ctl->eroded_total, 1ULL);
+    if (force_element<KIND, false>(D, s)) atomicAdd(&D.ctl->eroded_total, 1ULL);
 }
 
 __global__ __launch_bounds__(256) void k_gather_update(InstDev D) {
