@@ -332,6 +332,15 @@ def run(model: Model) -> Results:
     return to_results(r, u, v, m, g, model.settings.pulse_dt)
 
 
+def run_noalloc(model: Model) -> Results:
+    """run() with no device memory allocated: the solve's device buffers are
+    carved from the library's static device scratch (crush_run_noalloc)."""
+    lib = load()
+    r, u, v, m, g = alloc_model_results(model)
+    check(lib.crush_run_noalloc(model.ptr, C.byref(r)))
+    return to_results(r, u, v, m, g, model.settings.pulse_dt)
+
+
 def debug_run_steps(model: Model, n_steps: int):
     """Per-step taps of the first n_steps on the GPU: f_int, f_ext, hit,
     eroded, dt (see crush_debug_run_steps)."""

@@ -291,6 +291,7 @@ SIGNATURES = {
     "crush_last_error": (C.c_char_p, []),
     "crush_device_count": (C.c_int32, []),
     "crush_run": (C.c_int32, [C.POINTER(ModelView), C.POINTER(Results)]),
+    "crush_run_noalloc": (C.c_int32, [C.POINTER(ModelView), C.POINTER(Results)]),
     "crush_debug_run_steps": (C.c_int32, [C.POINTER(ModelView), C.c_int32, f64p, f64p, f64p, i32p, u8p, f64p, i32p]),
     "crush_contact_node_count": (C.c_int32, [C.POINTER(ModelView), i32p]),
     "crush_output_capacity": (C.c_int32, [C.POINTER(ModelView), i32p, i32p]),

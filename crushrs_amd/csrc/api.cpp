@@ -37,6 +37,13 @@ int32_t crush_run(const crush_model* model, crush_results* out) {
     });
 }
 
+int32_t crush_run_noalloc(const crush_model* model, crush_results* out) {
+    return guarded([&] {
+        if (!model) fail(CRUSH_E_INPUT, "null model");
+        run_plain_noalloc(*model, out);
+    });
+}
+
 int32_t crush_debug_run_steps(const crush_model* model, int32_t n_steps, double* f_int, double* f_ext, double* u,
                               int32_t* hit, uint8_t* eroded, double* dt, int32_t* steps_done) {
     return guarded([&] {

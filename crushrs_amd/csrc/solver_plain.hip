@@ -8,7 +8,9 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -335,18 +337,25 @@ static unsigned nblk(long long n, int b = 256) { return (unsigned)std::max<long 
 struct PlainSolver {
     const Prepared& P;
     hipStream_t stream = nullptr;
+    bool own_stream = false;
     std::unique_ptr<DeviceModel> D;
     long long host_step = 0;
     double cell_scale = 1.0;
 
-    explicit PlainSolver(const Prepared& p) : P(p) {
+    // With an arena, the device buffers come from it and the solve runs on
+    // the null stream (nothing is created); otherwise on a stream of its own.
+    explicit PlainSolver(const Prepared& p, DevArena* arena = nullptr) : P(p) {
         require_device();
-        HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        if (!arena) {
+            HIP_OK(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+            own_stream = true;
+        }
+        ArenaScope scope(arena);
         D.reset(new DeviceModel(P, stream));
     }
     ~PlainSolver() {
         D.reset();
-        if (stream) (void)hipStreamDestroy(stream);
+        if (own_stream) (void)hipStreamDestroy(stream);
     }
 
     // Launch one step (the kernels no-op once the solve is done).
@@ -431,15 +440,23 @@ static void fill_results(const Prepared& P, PlainSolver& S, const StepCtl& c, cr
 // Guard against a solve that never reaches end_time (dt is floored at 1e-15).
 static const long long kMaxSteps = 100000000LL;
 
-void run_plain(const crush_model& m, crush_results* out) {
+// arena: device buffers from it, no events (device_ms is then the host's
+// time around the step loop, which ends with a synchronisation)
+static void run_plain_in(const crush_model& m, crush_results* out, DevArena* arena) {
     auto t0 = std::chrono::steady_clock::now();
     Prepared P;
     prepare(m, P);
-    PlainSolver S(P);
-    hipEvent_t e0, e1;
-    HIP_OK(hipEventCreate(&e0));
-    HIP_OK(hipEventCreate(&e1));
-    HIP_OK(hipEventRecord(e0, S.stream));
+    if (arena && P.settings.frame_steps > 0 && out && out->frame_capacity > 0)
+        fail(CRUSH_E_UNSUPPORTED, "crush_run_noalloc records no frames (frame_steps > 0): frames need device "
+                                  "memory beyond its scratch; use crush_run");
+    PlainSolver S(P, arena);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!arena) {
+        HIP_OK(hipEventCreate(&e0));
+        HIP_OK(hipEventCreate(&e1));
+        HIP_OK(hipEventRecord(e0, S.stream));
+    }
+    const auto t_loop = std::chrono::steady_clock::now();
     const int chunk = 32;
     // frames: the initial state, every frame_steps steps, the last step
     const int fs = P.settings.frame_steps;
@@ -464,12 +481,17 @@ void run_plain(const crush_model& m, crush_results* out) {
             ++n_frames;
         }
     }
-    HIP_OK(hipEventRecord(e1, S.stream));
-    HIP_OK(hipEventSynchronize(e1));
     float ms = 0.0f;
-    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
+    if (!arena) {
+        HIP_OK(hipEventRecord(e1, S.stream));
+        HIP_OK(hipEventSynchronize(e1));
+        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    } else {
+        HIP_OK(hipStreamSynchronize(S.stream));
+        ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_loop).count();
+    }
     if (c.error) fail(CRUSH_E_NONFINITE, "displacement became non-finite at step %lld (t = %.3e)", c.step, c.t);
     if (out) {
         fill_results(P, S, c, out);
@@ -478,6 +500,53 @@ void run_plain(const crush_model& m, crush_results* out) {
         out->device_ms = ms;
         out->wall_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
+}
+
+void run_plain(const crush_model& m, crush_results* out) { run_plain_in(m, out, nullptr); }
+
+// The static device scratch of crush_run_noalloc: every device buffer of a
+// solve is carved from it.  Each GPU the library runs on holds its own copy
+// for the life of the process.  In host code the array's name is not a
+// device address: its address on the current device comes from
+// hipGetSymbolAddress, once per device.
+constexpr size_t kNoAllocBytes = (size_t)CRUSH_NOALLOC_SCRATCH_BYTES;
+static __device__ __align__(256) uint8_t scratch[kNoAllocBytes];
+
+namespace {
+struct NoAllocDevice {
+    uint8_t* ptr = nullptr;  // device address of `scratch` on this device
+    std::mutex run;          // calls on one device take turns
+};
+std::mutex g_noalloc_mu;
+std::map<int, std::unique_ptr<NoAllocDevice>> g_noalloc;
+
+// The current device's scratch (its address resolved on first use).
+NoAllocDevice& noalloc_device() {
+    int dev = 0;
+    HIP_OK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    std::unique_ptr<NoAllocDevice>& d = g_noalloc[dev];
+    if (!d) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            fail(CRUSH_E_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", dev, hipGetErrorString(e));
+        d.reset(new NoAllocDevice);
+        d->ptr = static_cast<uint8_t*>(p);
+    }
+    return *d;
+}
+}  // namespace
+
+void run_plain_noalloc(const crush_model& m, crush_results* out) {
+    require_device();
+    NoAllocDevice& d = noalloc_device();
+    std::lock_guard<std::mutex> lk(d.run);
+    DevArena arena;
+    arena.base = d.ptr;
+    arena.cap = kNoAllocBytes;
+    arena.owner = "crush_run_noalloc";
+    run_plain_in(m, out, &arena);
 }
 
 void run_plain_taps(const crush_model& m, int n_steps, double* f_int, double* f_ext, double* u_out, int32_t* hit,

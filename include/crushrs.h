@@ -244,6 +244,24 @@ int32_t crush_device_count(void);
  * Replaces crushrs::run (src/solver.rs:190-381, src/lib.rs:26). */
 int32_t crush_run(const crush_model* model, crush_results* out);
 
+/* Size of the static device scratch of crush_run_noalloc: 64 MiB per GPU,
+ * held from library load until the process ends. */
+#define CRUSH_NOALLOC_SCRATCH_BYTES (64u << 20)
+
+/* crush_run with no device memory allocated: every device buffer of the
+ * solve (the prepared model, nodal and element state, contact tables) is
+ * carved from a static device array of CRUSH_NOALLOC_SCRATCH_BYTES, and
+ * the solve runs on the null stream with no streams or events created.
+ * No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (tight serving loops, harnesses that forbid hipMalloc).
+ * Results are those of crush_run.  A model whose buffers do not fit
+ * (about 150k hexes) is refused with CRUSH_E_INPUT before any kernel runs;
+ * frames (settings.frame_steps > 0) are refused with CRUSH_E_UNSUPPORTED.
+ * Every call on one GPU uses that GPU's one array: calls from several host
+ * threads on one GPU take turns (a lock in the library), and the array
+ * keeps the last call's contents. */
+int32_t crush_run_noalloc(const crush_model* model, crush_results* out);
+
 /* Parity taps: run the first n_steps steps of the solve on the GPU and
  * record, per step s (0-based, after the step), f_int[s*3N..] (internal
  * nodal forces at the new configuration, src/solver.rs:327-328),
