@@ -5,6 +5,8 @@ error < 1e-4; contact-pair and erosion decisions identical for the first
 100 steps; per-vehicle delta-v within 0.05 m/s and crush within 2 mm at the
 end of the solve.
 """
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -16,28 +18,97 @@ pytestmark = pytest.mark.gpu
 FORCE_RTOL = 1e-4
 
 
-def step_parity(model, n_steps):
+def contact_tables(model):
+    """Per contact: the nodes the static broad phase keeps
+    (src/contact.rs:48-70), the face node ids and max_distance."""
+    v = model.view
+    X = np.ctypeslib.as_array(v.nodes, shape=(v.n_nodes, 3))
+    faces = np.ctypeslib.as_array(v.faces, shape=(v.n_faces, 4))
+    out = []
+    for c in range(v.n_contacts):
+        ct = v.contacts[c]
+        nodes = np.ctypeslib.as_array(ct.nodes, shape=(ct.n_nodes,))
+        fn = faces[np.ctypeslib.as_array(ct.faces, shape=(ct.n_faces,))]
+        lo = X[fn.ravel()].min(0) - ct.max_distance
+        hi = X[fn.ravel()].max(0) + ct.max_distance
+        keep = nodes[np.all((X[nodes] >= lo) & (X[nodes] <= hi), axis=1)]
+        out.append((keep, fn.copy(), ct.max_distance))
+    return out
+
+
+def decision_margin(x4, p, max_d):
+    """How far node p is from flipping its acceptance by the face with
+    corners x4 (src/contact.rs:116-128): the smallest distance, in metres,
+    from the AABB faces, the signed-distance window ends and the edges of
+    the best of the four triangles (barycentric margin x face size)."""
+    def tri(q, a, b, c, tol=0.1):
+        v0, v1, v2 = c - a, b - a, q - a
+        d00, d01, d02, d11, d12 = v0 @ v0, v0 @ v1, v0 @ v2, v1 @ v1, v1 @ v2
+        inv = 1.0 / (d00 * d11 - d01 * d01)
+        u = (d11 * d02 - d01 * d12) * inv
+        w = (d00 * d12 - d01 * d02) * inv
+        return min(u + tol, w + tol, 1.0 + tol - u - w)
+    nrm = np.cross(x4[1] - x4[0], x4[2] - x4[0])
+    nrm /= np.linalg.norm(nrm)
+    lo, hi = x4.min(0) - max_d, x4.max(0) + max_d
+    signed = -nrm @ (x4.mean(0) - p)
+    q = p - nrm * signed
+    size = np.linalg.norm(x4.max(0) - x4.min(0))
+    t = max(tri(q, x4[0], x4[1], x4[2]), tri(q, x4[0], x4[2], x4[3]), tri(q, x4[1], x4[2], x4[3]),
+            tri(q, x4[1], x4[3], x4[0]))
+    return min(np.abs(np.concatenate([p - lo, hi - p])).min(), abs(signed), abs(signed + max_d), abs(t) * size)
+
+
+def step_parity(model, n_steps, flip_tol=0.0):
+    """Per-step parity over the first n_steps.  With flip_tol > 0 a contact
+    decision may differ only where the node lies within flip_tol metres of
+    a decision boundary of a face either side picked (at the oracle's
+    positions the step starts from), at most once in 10^4 decisions; the
+    force and step bars then hold up to the first such flip that changes
+    the node's contact force beyond the f_ext bar (the two solves take
+    different, equally valid, branches from there), which must come in the
+    second half of the window."""
     k = model.contact_node_count()
     gi, gx, gh, ge, gdt = ca.debug_run_steps(model, n_steps)
     oi, ox, oh, oe, odt = of.run_steps(model, n_steps, k)
     assert len(gdt) == len(odt) == n_steps
     assert np.array_equal(ge, oe), "erosion decisions differ"
     mism = np.argwhere(gh != oh)
+    xn = np.linalg.norm(ox, axis=1)
+    xpeak = max(xn.max(), 1e-30)
+    cut = n_steps
+    if flip_tol > 0.0 and mism.size:
+        assert len(mism) <= 1e-4 * gh.size, f"{len(mism)} contact decisions differ"
+        tabs = contact_tables(model)
+        kept = [(c, n) for c, (keep, _, _) in enumerate(tabs) for n in keep]
+        X = np.ctypeslib.as_array(model.view.nodes, shape=(model.n_nodes, 3))
+        ou = np.zeros((n_steps, 3 * model.n_nodes))
+        done = C.c_int32()
+        assert of.load().oracle_run_displacements(model.ptr, n_steps, 0, of.ptr(ou), C.byref(done)) == 0
+        for s, j in mism:
+            c, node = kept[j]
+            pos = X + (ou[s - 1].reshape(-1, 3) if s > 0 else 0.0)
+            _, fn, md = tabs[c]
+            margins = [decision_margin(pos[fn[f]], pos[node], md) for f in {int(gh[s, j]), int(oh[s, j])} if f >= 0]
+            assert min(margins) < flip_tol, f"step {s} node {node}: decision differs {margins} m from a boundary"
+            df = np.linalg.norm(gx[s, 3 * node:3 * node + 3] - ox[s, 3 * node:3 * node + 3])
+            if df >= 1e-3 * max(xn[s], 1e-3 * xpeak):
+                cut = min(cut, int(s))
+        assert cut >= n_steps // 2, f"a flip changes a contact force at step {cut}"
+        mism = mism[:0]
     assert mism.size == 0, f"contact decisions differ at (step, node) {mism[:5].tolist()}"
     norms = np.linalg.norm(oi, axis=1)
     peak = norms.max()
-    for s in range(n_steps):
+    for s in range(cut):
         err = np.linalg.norm(gi[s] - oi[s])
         if norms[s] < 1e-3 * peak:
             # rigid-motion steps: the oracle's f_int is round-off of a zero force
             assert err < FORCE_RTOL * peak, f"step {s}: |df| {err:.3e} vs peak {peak:.3e}"
         else:
             assert err < FORCE_RTOL * norms[s], f"step {s}: f_int rel err {err / norms[s]:.2e}"
-    np.testing.assert_allclose(gdt, odt, rtol=1e-5)
+    np.testing.assert_allclose(gdt[:cut], odt[:cut], rtol=1e-5)
     # contact forces follow from identical decisions at nearly equal states
-    xn = np.linalg.norm(ox, axis=1)
-    xpeak = max(xn.max(), 1e-30)
-    for s in range(n_steps):
+    for s in range(cut):
         assert np.linalg.norm(gx[s] - ox[s]) < 1e-3 * max(xn[s], 1e-3 * xpeak)
     return gi, oi
 

@@ -1,7 +1,7 @@
 """GPU parity on the large single instance and the calibration set.
 
 The large instance (76,662 hexes) is held to the oracle per step over its
-first steps at full size, and its whole solve to the size-independent
+first 100 steps at full size, and its whole solve to the size-independent
 properties of a head-on (tests/impact.rs:33-47: momentum exchanged only
 through contact, delta-v between the perfectly plastic limit and +30 %,
 restitution in (0, 0.3)); the batched and one-model paths must agree
@@ -21,9 +21,16 @@ from envelope import oracle_envelope, within_envelope
 pytestmark = pytest.mark.gpu
 
 
-def test_large_instance_first_steps_match_oracle():
-    print("case: large pair 0.075 m, 12 steps vs oracle", flush=True)
-    step_parity(workloads.large_pair(), 12)
+def test_large_instance_first_100_steps_match_oracle():
+    """Per step for 100 steps at full size.  Of its ~118k contact decisions
+    a few flip between the f32 GPU and the f64 oracle where a node lies on
+    a face's plane to within the two trajectories' difference (signed
+    distances of 1e-8 to 2e-7 m at positions 1e-8 to 3e-7 m apart): those
+    are accepted only within 1e-6 m of a decision boundary.  One of them
+    (step 97) sends a node to the next face in order at 0.9 mm depth, so
+    the force bars hold up to that step."""
+    print("case: large pair 0.075 m, 100 steps vs oracle", flush=True)
+    step_parity(workloads.large_pair(), 100, flip_tol=1e-6)
 
 
 def test_large_instance_solve_properties_and_batch_equals_plain():
