@@ -1,10 +1,11 @@
 // Batched solves: thousands of independent models on one GPU.
 //
 // Each instance is one contiguous image in a device pool (its arrays plus a
-// control block) described by an InstDev pointer table.  One explicit step
-// of every active instance is nine launches (ten with a large contact),
-// each over all instances:
-//   k_end              one workgroup per instance: end of the previous step
+// control block) described by an InstDev pointer table.  One tick of every
+// active instance is eight launches (nine with a large contact), each over
+// all instances; a tick gathers one step and runs the contact, half step
+// and forces of the next:
+//   k_ctl              one workgroup per instance: end of the gathered step
 //                      and the stable step of the next (at check steps);
 //   k_facegeo_tiles    face geometry of contacts of more than 256 moving
 //                      faces, in tiles of faces;
@@ -15,12 +16,15 @@
 //                      nodes); k_faceforce_tiles face reactions (tiles of
 //                      faces); k_apply_tiles contact forces (tiles of
 //                      touched nodes);
-//   k_half_tiles       first half step (tiles of 256 nodes);
+//   k_node_tiles       node-centric gather and second half step of the
+//                      step in flight, fused with the first half step of
+//                      the next (tiles of 256 nodes);
 //   k_force_hc,        element forces and the next check step's stable-step
 //   k_force_el         partials (tiles of element slots), honeycomb and
 //                      elastic tiles in separate launches so each fills the
-//                      GPU at its own register budget;
-//   k_gather_tiles     node-centric gather and second half step.
+//                      GPU at its own register budget.
+// A step whose end reads the full-step velocity (the last step, history
+// samples, separation checks) is gathered in a tick of its own.
 // No arithmetic of an instance depends on the others, so its results are
 // bitwise the same alone or in any mix.  A fixed number of steps is
 // captured in one hipGraph and replayed until every instance has retired;
@@ -87,6 +91,18 @@ static StepCtl initial_ctl(const Prepared& P) {
     return c;
 }
 
+// Probe nodes of an instance, ascending: each accelerometer node of groups
+// < 8 and its frame's origin and x node.
+static std::vector<int32_t> probe_nodes(const Prepared& P) {
+    std::vector<int32_t> v;
+    for (int g = 0; g < std::min(P.n_groups, 8); ++g)
+        for (int n : {P.g_accel[g], P.g_frame[3 * g], P.g_frame[3 * g + 1]})
+            if (n >= 0) v.push_back(n);
+    std::sort(v.begin(), v.end());
+    v.erase(std::unique(v.begin(), v.end()), v.end());
+    return v;
+}
+
 // Stable-step partial slots of one instance (enough for the smallest tile).
 static size_t dtp_slots(const Prepared& P) { return (size_t)(P.n_hc + 63) / 64 + (P.n_elems - P.n_hc + 63) / 64 + 2; }
 
@@ -115,6 +131,7 @@ ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     o.v = b.put(P.v0);
     o.f_int = b.zeros(sizeof(double) * 3 * N);
     o.f_ext = b.zeros(sizeof(double) * 3 * N);
+    o.f_ext2 = b.zeros(sizeof(double) * 3 * N);
     {
         std::vector<double> im(N);
         for (int i = 0; i < N; ++i) im[i] = P.mass[i] > 0.0 ? 1.0 / P.mass[i] : 0.0;
@@ -122,7 +139,26 @@ ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     }
     o.mass = b.put(P.mass);
     o.added = b.zeros(sizeof(double) * N);
-    o.fixed = b.put(P.fixed);
+    const std::vector<int32_t> probes = probe_nodes(P);
+    {
+        std::vector<uint8_t> fl = P.fixed;
+        for (int32_t n : probes) fl[n] |= NODE_PROBE;
+        o.fixed = b.put(fl);
+    }
+    o.probe_node = b.put(probes);
+    {
+        auto slot = [&](int n) {
+            return n < 0 ? -1 : (int32_t)(std::lower_bound(probes.begin(), probes.end(), n) - probes.begin());
+        };
+        std::vector<int32_t> gp;
+        for (int g = 0; g < std::min(P.n_groups, 8); ++g) {
+            const int na = P.g_accel[g];
+            gp.push_back(na >= 0 && !(P.fixed[na] & NODE_FIXED) ? slot(na) : -1);
+            gp.push_back(slot(P.g_frame[3 * g]));
+            gp.push_back(slot(P.g_frame[3 * g + 1]));
+        }
+        o.g_probe = b.put(gp);
+    }
     o.n2e_ptr = b.put(P.n2e_ptr);
     o.n2e_idx = b.put(P.n2e_idx);
     // face-grid region of each contact: [starts: cells + 1][entries], plus a
@@ -187,6 +223,7 @@ ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     o.pulse = scratch(sizeof(double) * (size_t)std::min(P.n_groups, 8) * (size_t)P.n_pulse);
     // history records, written in order as the samples are taken
     o.hist = scratch(sizeof(double) * hist_record_len(P.n_parts, P.n_groups) * (size_t)P.n_hist);
+    o.probe = scratch(sizeof(double) * 6 * probes.size());
     o.total = off;
     out.swap(b.bytes);
     return o;
@@ -220,6 +257,7 @@ InstDev make_inst(const Prepared& P, const ImageOffsets& o, uintptr_t base) {
     I.v = at<double>(base, o.v);
     I.f_int = at<double>(base, o.f_int);
     I.f_ext = at<double>(base, o.f_ext);
+    I.f_ext2 = at<double>(base, o.f_ext2);
     I.inv_mass = at<double>(base, o.inv_mass);
     I.mass = at<double>(base, o.mass);
     I.added = at<double>(base, o.added);
@@ -268,6 +306,10 @@ InstDev make_inst(const Prepared& P, const ImageOffsets& o, uintptr_t base) {
     I.cgp = at<FaceGrid>(base, o.cgp);
     I.dtp_min = at<double>(base, o.dtp_min);
     I.dtp_sum = at<double>(base, o.dtp_sum);
+    I.probe_node = at<int32_t>(base, o.probe_node);
+    I.probe = at<double>(base, o.probe);
+    I.n_probe = (int)probe_nodes(P).size();
+    I.g_probe = at<int32_t>(base, o.g_probe);
     return I;
 }
 
@@ -315,16 +357,35 @@ __device__ __forceinline__ void dt_partial(const InstDev& D, double dt_floor, in
     }
 }
 
-// Per-instance part of a step (src/solver.rs:287-352), one workgroup per
-// active instance.  It closes the open step (end-of-step bookkeeping after
-// the gather) and opens the next with its stable step (from the tile
-// partials, at check steps).  `pending` marks an open step whose contact,
-// half step, element forces and gather are due (the tile kernels below).
+// Control pass of a tick (src/solver.rs:287-352), one workgroup per active
+// instance.  A tick runs, per instance, the contact, first half step and
+// element forces of one step (PH_HALF) and the gather of the step before it
+// (PH_GATHER); a gather and the next half step share one node pass, which
+// keeps the gathered forces and the full-step velocity in registers.  The
+// pass
+//  (a) after a gather alone, closes that step in full (step_end); after a
+//      gather fused with the next half step, streams the step's pulse
+//      sample from the probe records, and stops the instance if that gather
+//      left a non-finite velocity;
+//  (b) holds the step in flight (half step and forces done) for a gather
+//      alone when its end reads the full-step state: the solve may end
+//      there, it takes a history sample or it checks separation.  Any other
+//      step is closed now as far as its displacement allows (time, step
+//      count, crush, contact and erosion counts);
+//  (c) opens the next step with its stable step (from the force tiles'
+//      partials, at check steps).
+// Every step thus runs the same arithmetic, in the same order per node, as
+// the one-solve path.  One difference: when a fused gather leaves a
+// non-finite velocity, the solve stops at that step with the error flag,
+// but the velocities and displacements left behind (and the reported
+// delta-v, crush and last history record) are those after the next half
+// step.
 template <int B>
-__global__ __launch_bounds__(B) void k_end(const InstDev* __restrict__ table, const int32_t* __restrict__ list,
+__global__ __launch_bounds__(B) void k_ctl(const InstDev* __restrict__ table, const int32_t* __restrict__ list,
                                            const int32_t* __restrict__ n_list, int force_tile,
                                            uint8_t* __restrict__ done_flag, int* __restrict__ n_done) {
     __shared__ double s_w[B / 64];
+    __shared__ double s_a[8];
     __shared__ StepCtl s_ctl;
     if ((int)blockIdx.x >= *n_list) return;
     const int inst = list[blockIdx.x];
@@ -334,30 +395,98 @@ __global__ __launch_bounds__(B) void k_end(const InstDev* __restrict__ table, co
     if (t == 0) *c = *D.ctl;
     __syncthreads();
     if (c->done) return;
-    if (c->pending) {
+    const int ph = c->phase;
+    const int ng = D.n_groups < 8 ? D.n_groups : 8;
+    // (a) the last tick's gather
+    if (ph & PH_GATHER) {
+        bool stop;
+        if (ph & PH_HALF) {
+            stop = (c->bad_step & BAD_GATHER) != 0;
+            if (stop) {
+                const int hk = c->hist_k;
+                if (D.n_hist > 0 && hk < D.n_hist) history_sample<B>(D, hk, c->step, c->t, s_w, fext_step(D, c->step));
+            }
+            // thread g forms group g's sample
+            if (D.n_pulse > 0 && t < ng) s_a[t] = accel_local_x_probed(D, t);
+            __syncthreads();
+            if (t == 0) {
+                if (D.n_pulse > 0) pulse_sample(D, c, c->t, s_a);
+                if (stop) {
+                    if (D.n_hist > 0) c->hist_k += 1;
+                    c->error = 1;
+                    c->done = 1;
+                }
+            }
+            __syncthreads();
+        } else {
+            const double dt = step_dt(*c);
+            const int hits = c->pair_hits;
+            const bool bad = c->bad_step != 0;
+            double* fext = fext_step(D, c->step + 1);
+            __syncthreads();
+            if (t == 0) {
+                c->eroded_total += (unsigned long long)c->eroded_step;
+                c->eroded_step = 0;
+                c->bad_step = 0;
+                c->pair_hits = 0;
+            }
+            __syncthreads();
+            stop = step_end<B>(D, c, dt, hits, bad, s_w, fext);
+        }
+        if (stop) {
+            finish_instance<B>(D, c, s_w);
+            __syncthreads();
+            if (t == 0) {
+                c->phase = 0;
+                *D.ctl = *c;
+                done_flag[inst] = 1;
+                atomicAdd(n_done, 1);
+            }
+            return;
+        }
+    }
+    // (b) the step in flight
+    bool closed = false;
+    if (ph & PH_HALF) {
+        const long long q = c->step + 1;
         const double dt = step_dt(*c);
         const int hits = c->pair_hits;
-        const bool bad = c->bad_step != 0;
+        const int had_pair = hits > 0 ? 1 : c->had_pair;
+        const int quiet = hits > 0 ? 0 : c->quiet + 1;
+        const double tn = c->t + dt;
+        const bool hold = (c->bad_step & BAD_HALF) != 0 || !(tn < c->end_time) ||
+                          (D.n_hist > 0 && q % D.hist_steps == 0) ||
+                          (c->sep_steps > 0 && D.n_groups >= 2 && had_pair && quiet >= c->sep_steps && q % c->chk == 0);
+        if (hold) {
+            __syncthreads();
+            if (t == 0) {
+                c->phase = PH_GATHER;
+                *D.ctl = *c;
+            }
+            return;
+        }
+        double cr[8];
+        for (int g = 0; g < ng; ++g) cr[g] = group_crush<B>(D, g, s_w);
         __syncthreads();
         if (t == 0) {
+            for (int g = 0; g < ng; ++g) c->crush_max[g] = fmax(c->crush_max[g], cr[g]);
             c->eroded_total += (unsigned long long)c->eroded_step;
             c->eroded_step = 0;
             c->bad_step = 0;
             c->pair_hits = 0;
-            c->pending = 0;
+            c->had_pair = had_pair;
+            c->quiet = quiet;
+            c->t = tn;
+            c->dt = dt;
+            c->step = q;
         }
         __syncthreads();
-        if (step_end<B>(D, c, dt, hits, bad, s_w)) {
-            finish_instance<B>(D, c, s_w);
-            if (t == 0) {
-                done_flag[inst] = 1;
-                atomicAdd(n_done, 1);
-            }
-        }
+        closed = true;
     }
-    // stable step and selective mass scaling (src/solver.rs:288-299) from
-    // the force tiles' partials, reduced by the whole block
-    const bool check = !c->done && c->adaptive && c->step % c->chk == 0;
+    // (c) stable step and selective mass scaling of the next step
+    // (src/solver.rs:288-299) from the force tiles' partials, reduced by the
+    // whole block
+    const bool check = c->adaptive && c->step % c->chk == 0;
     double mn = INFINITY, sm = 0.0;
     if (check) {
         const int T = force_tile;
@@ -369,7 +498,7 @@ __global__ __launch_bounds__(B) void k_end(const InstDev* __restrict__ table, co
         mn = block_reduce<B>(mn, 1, s_w);
         sm = block_reduce<B>(sm, 0, s_w);
     }
-    if (t == 0 && !c->done) {
+    if (t == 0) {
         c->scale_now = 0;
         if (check) {
             if (isfinite(mn)) {
@@ -381,10 +510,9 @@ __global__ __launch_bounds__(B) void k_end(const InstDev* __restrict__ table, co
                 c->dt_cfl = c->dt_scale * fmin(scale ? c->dt_floor : mn, c->contact_dt);
             }
         }
-        c->pending = 1;
+        c->phase = PH_HALF | (closed ? PH_GATHER : 0);
+        *D.ctl = *c;
     }
-    __syncthreads();
-    if (t == 0) *D.ctl = *c;
 }
 
 // Contact (src/contact.rs:92-160), in four or five launches over all open
@@ -404,7 +532,7 @@ __global__ __launch_bounds__(B) void k_grid(const InstDev* __restrict__ table, c
     if ((int)blockIdx.x >= *n_tiles) return;
     const int2 tile = tiles[blockIdx.x];
     const InstDev& D = table[tile.x];
-    if (!D.ctl->pending) return;
+    if (!(D.ctl->phase & PH_HALF)) return;
     const int ci = tile.y;
     const ContactDesc cd = D.cdesc[ci];
     // faces of fixed nodes never move
@@ -452,7 +580,7 @@ __global__ __launch_bounds__(T) void k_facegeo_tiles(const InstDev* __restrict__
     if ((int)blockIdx.x >= *n_tiles) return;
     const int2 tile = tiles[blockIdx.x];
     const InstDev& D = table[tile.x];
-    if (!D.ctl->pending) return;
+    if (!(D.ctl->phase & PH_HALF)) return;
     const int ci = tile.y >> 20;
     const ContactDesc cd = D.cdesc[ci];
     const int fl = (tile.y & 0xFFFFF) + (int)threadIdx.x;
@@ -475,7 +603,7 @@ __global__ __launch_bounds__(T * L) void k_search_tiles(const InstDev* __restric
     const int2 tile = tiles[blockIdx.x];
     const InstDev& D = table[tile.x];
     StepCtl& c = *D.ctl;
-    if (!c.pending) return;
+    if (!(c.phase & PH_HALF)) return;
     int ci = 0;
     while (ci + 1 < D.n_contacts && D.cdesc[ci + 1].node_begin <= tile.y) ++ci;
     const ContactDesc cd = D.cdesc[ci];
@@ -499,7 +627,7 @@ __global__ __launch_bounds__(T) void k_faceforce_tiles(const InstDev* __restrict
     if ((int)blockIdx.x >= *n_tiles) return;
     const int2 tile = tiles[blockIdx.x];
     const InstDev& D = table[tile.x];
-    if (!D.ctl->pending) return;
+    if (!(D.ctl->phase & PH_HALF)) return;
     const int f = tile.y + (int)threadIdx.x;
     if (f < D.F) contact_face_force(D, f);
 }
@@ -510,26 +638,10 @@ __global__ __launch_bounds__(T) void k_apply_tiles(const InstDev* __restrict__ t
     if ((int)blockIdx.x >= *n_tiles) return;
     const int2 tile = tiles[blockIdx.x];
     const InstDev& D = table[tile.x];
-    if (!D.ctl->pending) return;
-    const int i = tile.y + (int)threadIdx.x;
-    if (i < D.T) contact_apply_node(D, i);
-}
-
-// First half step of every node of the open steps (src/solver.rs:313-323),
-// after the scaling mass of the node at steps that scale (src/kernel.rs:
-// 243-258): one workgroup per tile of T nodes of one instance.
-template <int T>
-__global__ __launch_bounds__(T) void k_half_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
-                                                  const int32_t* __restrict__ n_tiles) {
-    if ((int)blockIdx.x >= *n_tiles) return;
-    const int2 tile = tiles[blockIdx.x];
-    const InstDev& D = table[tile.x];
     const StepCtl& c = *D.ctl;
-    if (!c.pending) return;
-    const int n = tile.y + (int)threadIdx.x;
-    if (n >= D.N) return;
-    if (c.scale_now) mass_node(D, n);
-    if (!half_node(D, n, step_dt(c))) atomicOr(&D.ctl->bad_step, 1);
+    if (!(c.phase & PH_HALF)) return;
+    const int i = tile.y + (int)threadIdx.x;
+    if (i < D.T) contact_apply_node(D, i, fext_step(D, c.step + 1));
 }
 
 // Element forces of the open steps: one workgroup per tile of T element
@@ -543,7 +655,7 @@ __device__ __forceinline__ void force_tile(const InstDev* __restrict__ table, co
     const int2 tile = tiles[blockIdx.x];
     const InstDev& D = table[tile.x];
     StepCtl& c = *D.ctl;
-    if (!c.pending) return;
+    if (!(c.phase & PH_HALF)) return;
     const int end = KIND == CRUSH_MAT_HONEYCOMB ? D.n_hc : D.E;
     const int s = tile.y + (int)threadIdx.x;
     int er = 0;
@@ -584,25 +696,100 @@ __global__ __launch_bounds__(T) void k_dt_tiles(const InstDev* __restrict__ tabl
     const int2 tile = tiles[blockIdx.x];
     const InstDev& D = table[tile.x];
     const StepCtl& c = *D.ctl;
-    if (c.done || c.pending || c.step != 0 || !c.adaptive) return;
+    if (c.done || c.phase != 0 || c.step != 0 || !c.adaptive) return;
     const int end = KIND == CRUSH_MAT_HONEYCOMB ? D.n_hc : D.E;
     dt_partial<T>(D, c.dt_floor, tile.y + (int)threadIdx.x, end, dt_slot<KIND, T>(D, tile.y), s_w);
 }
 
-// Gather of the element forces and second half step of the open steps
-// (src/solver.rs:327-342).
+// Node pass of a tick, tiles of T nodes of one instance: the gather and
+// second half step of the step in flight (src/solver.rs:327-342) and the
+// first half step of the next (src/solver.rs:313-323), after the node's
+// scaling mass at steps that scale (src/kernel.rs:243-258).  Fused, the
+// gathered force and full-step velocity stay in registers and only the
+// half-step velocity and the displacement are written; the probe nodes
+// record their acceleration and position for the pulses.  A gather alone
+// writes f_int and v, which a half step alone (an instance's first step, or
+// the step after a gather alone) reads.
 template <int T>
-__global__ __launch_bounds__(T) void k_gather_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
-                                                    const int32_t* __restrict__ n_tiles) {
+__global__ __launch_bounds__(T) void k_node_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
+                                                  const int32_t* __restrict__ n_tiles) {
     if ((int)blockIdx.x >= *n_tiles) return;
     const int2 tile = tiles[blockIdx.x];
     const InstDev& D = table[tile.x];
     StepCtl& c = *D.ctl;
-    if (!c.pending) return;
+    const int ph = c.phase;
+    if (ph == 0) return;
     const int n = tile.y + (int)threadIdx.x;
     if (n >= D.N) return;
-    // (the displacement was checked by the half step)
-    if (!gather_node<false>(D, n, step_dt(c), c.damping)) atomicOr(&c.bad_step, 1);
+    const uint8_t fl = D.fixed[n];
+    const bool fx = (fl & NODE_FIXED) != 0;
+    const double dth = step_dt(c);
+    double im = D.inv_mass[n];
+    double f[3], vd[3];
+    {
+        const Double3 v = ld_d3(D.v, n);
+        vd[0] = v.x;
+        vd[1] = v.y;
+        vd[2] = v.z;
+    }
+    if (ph & PH_GATHER) {
+        // the step in flight: closed by the control pass when a half step
+        // follows, else still open
+        const bool fused = (ph & PH_HALF) != 0;
+        const long long qg = fused ? c.step : c.step + 1;
+        const double dtg = fused ? c.dt : dth;
+        gather_sum(D, n, f);
+        double fe[3] = {0.0, 0.0, 0.0};
+        if (fl & NODE_CONTACT) {
+            const Double3 e = ld_d3(fext_step(D, qg), n);
+            fe[0] = e.x;
+            fe[1] = e.y;
+            fe[2] = e.z;
+        }
+        double vn[3], a[3];
+        if (!gather_update(f, fe, im, dtg, c.damping, fx, vd, vn, a)) atomicOr(&c.bad_step, BAD_GATHER);
+        if (!fused) {
+            st_d3(D.f_int, n, Double3{f[0], f[1], f[2]});
+            st_d3(D.v, n, Double3{vn[0], vn[1], vn[2]});
+            return;
+        }
+        if (fl & NODE_PROBE) {
+            double* r = D.probe + 6 * (size_t)probe_slot(D, n);
+            double x[3];
+            node_position(D, n, x);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                r[d] = a[d];
+                r[3 + d] = x[d];
+            }
+        }
+#pragma unroll
+        for (int d = 0; d < 3; ++d) vd[d] = vn[d];
+    } else {
+        const Double3 fi = ld_d3(D.f_int, n);
+        f[0] = fi.x;
+        f[1] = fi.y;
+        f[2] = fi.z;
+    }
+    if (c.scale_now) im = mass_node(D, n);
+    if (fx) {
+        st_d3(D.u, n, Double3{0.0, 0.0, 0.0});
+        st_d3(D.v, n, Double3{0.0, 0.0, 0.0});
+        return;
+    }
+    double fe[3] = {0.0, 0.0, 0.0};
+    if (fl & NODE_CONTACT) {
+        const Double3 e = ld_d3(fext_step(D, c.step + 1), n);
+        fe[0] = e.x;
+        fe[1] = e.y;
+        fe[2] = e.z;
+    }
+    const Double3 u = ld_d3(D.u, n);
+    const double ud[3] = {u.x, u.y, u.z};
+    double vh[3], un[3];
+    if (!half_update(f, fe, im, dth, vd, ud, vh, un)) atomicOr(&c.bad_step, BAD_HALF);
+    st_d3(D.v, n, Double3{vh[0], vh[1], vh[2]});
+    st_d3(D.u, n, Double3{un[0], un[1], un[2]});
 }
 
 // Compact per-instance summary for the host.
@@ -733,7 +920,7 @@ static void launch_step(BatchImpl& bi, double cs, hipEvent_t* ev) {
         if (ev) HIP_OK(hipEventRecord(ev[k], st));
     };
     mark(K_END);
-    hipLaunchKernelGGL(k_end<B>, dim3(std::max(bi.cap[L_INST], 1)), dim3(B), 0, st, tab, bi.d_list.p, cnt + L_INST,
+    hipLaunchKernelGGL(k_ctl<B>, dim3(std::max(bi.cap[L_INST], 1)), dim3(B), 0, st, tab, bi.d_list.p, cnt + L_INST,
                        T, bi.done_flag.p, bi.n_done.p);
     mark(K_GRID);
     if (bi.cap[L_FGEO] > 0)
@@ -768,9 +955,9 @@ static void launch_step(BatchImpl& bi, double cs, hipEvent_t* ev) {
     if (bi.cap[L_APPLY] > 0)
         hipLaunchKernelGGL(k_apply_tiles<CONTACT_TILE>, dim3(bi.cap[L_APPLY]), dim3(CONTACT_TILE), 0, st, tab,
                            bi.d_tiles[L_APPLY].p, cnt + L_APPLY);
-    mark(K_HALF);
+    mark(K_NODE);
     if (bi.cap[L_NODE] > 0)
-        hipLaunchKernelGGL(k_half_tiles<NODE_TILE>, dim3(bi.cap[L_NODE]), dim3(NODE_TILE), 0, st, tab,
+        hipLaunchKernelGGL(k_node_tiles<NODE_TILE>, dim3(bi.cap[L_NODE]), dim3(NODE_TILE), 0, st, tab,
                            bi.d_tiles[L_NODE].p, cnt + L_NODE);
     mark(K_FORCE_HC);
     if (bi.cap[L_HC] > 0)
@@ -780,10 +967,6 @@ static void launch_step(BatchImpl& bi, double cs, hipEvent_t* ev) {
     if (bi.cap[L_EL] > 0)
         hipLaunchKernelGGL(k_force_el<T>, dim3(bi.cap[L_EL]), dim3(T), 0, st, tab,
                            bi.d_tiles[L_EL].p, cnt + L_EL);
-    mark(K_GATHER);
-    if (bi.cap[L_NODE] > 0)
-        hipLaunchKernelGGL(k_gather_tiles<NODE_TILE>, dim3(bi.cap[L_NODE]), dim3(NODE_TILE), 0, st, tab,
-                           bi.d_tiles[L_NODE].p, cnt + L_NODE);
     mark(N_KERNELS);
 }
 
@@ -1092,7 +1275,7 @@ void batch_run(BatchImpl& bi, crush_batch_stats* st) {
         const double* k = bi.t_kernel;
         st->k_end_ms = k[K_END];
         st->k_contact_ms = k[K_GRID] + k[K_SEARCH] + k[K_FACEFORCE] + k[K_APPLY];
-        st->k_node_ms = k[K_HALF] + k[K_GATHER];
+        st->k_node_ms = k[K_NODE];
         st->k_force_hc_ms = k[K_FORCE_HC];
         st->k_force_el_ms = k[K_FORCE_EL];
         for (int j = 0; j < N_KERNELS && j < 12; ++j) st->k_ms[j] = k[j];

@@ -35,9 +35,9 @@ struct StepCtl {
     int done, error, scale_now, had_pair;
     int quiet, separated, pair_hits, chk;
     int sep_steps, adaptive;
-    int pending;      // batched path: element forces of the open step are due
+    int phase;        // batched path: PH_* work of the current tick
     int eroded_step;  // batched path: elements eroded by the open step's force pass
-    int bad_step;     // batched path: a node of the open step became non-finite
+    int bad_step;     // batched path: BAD_* a node became non-finite
     int reserved3;
     double crush_max[8];
     double crush_end[8];
@@ -50,6 +50,15 @@ struct StepCtl {
     int pulse_k;
     int hist_k;  // history records taken after the initial one (may exceed the capacity)
 };
+
+// Batched path, per instance and tick (StepCtl::phase): PH_HALF runs the
+// contact, first half step and element forces of the step after the last
+// closed one; PH_GATHER gathers the forces of the step in flight (with
+// PH_HALF in the same node pass, before that half step).  StepCtl::bad_step:
+// the half step (BAD_HALF) or the gather (BAD_GATHER) left a non-finite
+// value.
+constexpr int PH_HALF = 1, PH_GATHER = 2;
+constexpr int BAD_HALF = 1, BAD_GATHER = 2;
 
 // Face geometry record width (doubles) in the SoA face array:
 // x0..x3 (12), normal (3), centre (3), inflated lo (3), hi (3),

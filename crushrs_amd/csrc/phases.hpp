@@ -94,7 +94,8 @@ struct InstDev {
     double* u;
     double* v;
     double* f_int;
-    double* f_ext;
+    double* f_ext;   // contact forces (batched path: of even steps)
+    double* f_ext2;  // batched path: contact forces of odd steps (one-solve path: f_ext)
     double* inv_mass;
     const double* mass;
     double* added;
@@ -151,7 +152,23 @@ struct InstDev {
     // grid_entries describe) and its parameters
     int32_t* cgrid;
     FaceGrid* cgp;
+    // batched path: nodes whose gathered state the pulses read (each
+    // accelerometer node and its frame's origin and x node, groups < 8,
+    // flagged NODE_PROBE), and per node a record {acceleration, position}
+    // written by the gather of every step whose gather runs with the next
+    // half step
+    const int32_t* probe_node;
+    double* probe;
+    int n_probe;
+    // per group < 8: record slots of the accelerometer node (-1: none, or
+    // a fixed node), the frame's origin and its x node (-1: no frame)
+    const int32_t* g_probe;
 };
+
+// Contact forces of 1-based step q (two buffers by the parity of q on the
+// batched path, where the contact of step q + 1 runs before the gather of
+// step q; one buffer on the one-solve path).
+__device__ __forceinline__ double* fext_step(const InstDev& D, long long q) { return (q & 1) ? D.f_ext2 : D.f_ext; }
 
 __device__ __forceinline__ double step_dt(const StepCtl& c) { return fmax(fmin(c.dt_cfl, c.end_time - c.t), 1e-15); }
 
@@ -179,14 +196,16 @@ __device__ __forceinline__ void dt_element(const InstDev& D, int s, double dt_fl
     *ex_out = ex;
 }
 
-// Added mass and inverse mass of one node.
-__device__ __forceinline__ void mass_node(const InstDev& D, int n) {
+// Added mass and inverse mass of one node (the inverse mass is returned).
+__device__ __forceinline__ double mass_node(const InstDev& D, int n) {
 #pragma clang fp contract(off)
     double a = 0.0;
     for (int k = D.n2e_ptr[n]; k < D.n2e_ptr[n + 1]; ++k) a += D.extra[D.n2e_idx[k] >> 3] / 8.0;
     D.added[n] = a;
     const double m = D.mass[n] + a;
-    D.inv_mass[n] = m > 0.0 ? 1.0 / m : 0.0;
+    const double im = m > 0.0 ? 1.0 / m : 0.0;
+    D.inv_mass[n] = im;
+    return im;
 }
 
 // Geometry of contact face f for the current configuration.
@@ -360,11 +379,11 @@ __device__ __forceinline__ void contact_face_force(const InstDev& D, int f) {
     D.fh_count[f] = 0;
 }
 
-// External (contact) force of touched node i, in contact order.  With
-// FUSED the reactions of the node's faces are formed here (the same
+// External (contact) force of touched node i, in contact order, into fext.
+// With FUSED the reactions of the node's faces are formed here (the same
 // arithmetic as contact_face_force) instead of read from face_force.
 template <bool FUSED = false>
-__device__ __forceinline__ void contact_apply_node(const InstDev& D, int i) {
+__device__ __forceinline__ void contact_apply_node(const InstDev& D, int i, double* fext) {
 #pragma clang fp contract(off)
     double s[3] = {0.0, 0.0, 0.0};
     for (int k = D.tn_ptr[i]; k < D.tn_ptr[i + 1]; ++k) {
@@ -387,37 +406,44 @@ __device__ __forceinline__ void contact_apply_node(const InstDev& D, int i) {
         }
     }
     const int n = D.tn_node[i];
-    st_d3(D.f_ext, n, Double3{s[0], s[1], s[2]});
+    st_d3(fext, n, Double3{s[0], s[1], s[2]});
 }
 
-// First half step of node n (src/solver.rs:313-323); v holds v_1/2 after.
-// Returns false when the new displacement is not finite.
-__device__ __forceinline__ bool half_node(const InstDev& D, int n, double dt) {
+// First half step of a free node (src/solver.rs:313-323): v_1/2 = v + dt/2
+// (f_ext - f_int) / m, u += dt v_1/2.  Returns false when the new
+// displacement is not finite.
+__device__ __forceinline__ bool half_update(const double* fi, const double* fe, double im, double dt, const double* v,
+                                            const double* u, double* vh, double* un) {
 #pragma clang fp contract(off)
+    bool ok = true;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const double a = (fe[d] - fi[d]) * im;
+        vh[d] = v[d] + 0.5 * dt * a;
+        un[d] = u[d] + dt * vh[d];
+        ok = ok && isfinite(un[d]);
+    }
+    return ok;
+}
+
+// First half step of node n with contact forces fext; v holds v_1/2 after
+// (fixed nodes: u = v = 0).  Returns false when the new displacement is not
+// finite.
+__device__ __forceinline__ bool half_node(const InstDev& D, int n, double dt, const double* fext) {
     const uint8_t fl = D.fixed[n];
     if (fl & NODE_FIXED) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) {
-            D.u[3 * (size_t)n + d] = 0.0;
-            D.v[3 * (size_t)n + d] = 0.0;
-        }
+        st_d3(D.u, n, Double3{0.0, 0.0, 0.0});
+        st_d3(D.v, n, Double3{0.0, 0.0, 0.0});
         return true;
     }
     const double im = D.inv_mass[n];
     // each nodal vector moves as one 24-byte access (16 + 8 bytes)
     const Double3 fi = ld_d3(D.f_int, n), v = ld_d3(D.v, n), u = ld_d3(D.u, n);
-    const Double3 fe = (fl & NODE_CONTACT) ? ld_d3(D.f_ext, n) : Double3{0.0, 0.0, 0.0};
+    const Double3 fe = (fl & NODE_CONTACT) ? ld_d3(fext, n) : Double3{0.0, 0.0, 0.0};
     const double fid[3] = {fi.x, fi.y, fi.z}, fed[3] = {fe.x, fe.y, fe.z};
     const double vd[3] = {v.x, v.y, v.z}, ud[3] = {u.x, u.y, u.z};
     double vh[3], un[3];
-    bool ok = true;
-#pragma unroll
-    for (int d = 0; d < 3; ++d) {
-        const double a = (fed[d] - fid[d]) * im;
-        vh[d] = vd[d] + 0.5 * dt * a;
-        un[d] = ud[d] + dt * vh[d];
-        ok = ok && isfinite(un[d]);
-    }
+    const bool ok = half_update(fid, fed, im, dt, vd, ud, vh, un);
     st_d3(D.v, n, Double3{vh[0], vh[1], vh[2]});
     st_d3(D.u, n, Double3{un[0], un[1], un[2]});
     return ok;
@@ -614,13 +640,11 @@ __device__ __forceinline__ int force_element(const InstDev& D, int s) {
     return force_element_class<KIND, false>(D, s, c);
 }
 
-// Node-centric gather of the element forces (f64 sums in element order)
-// and the second half step (src/solver.rs:327-342).  Returns false when the
-// node's state is not finite.
-template <bool CHECK_U = true>
-__device__ __forceinline__ bool gather_node(const InstDev& D, int n, double dt, double damp) {
+// Node-centric gather of the element forces into f (f64 sums in element
+// order; src/solver.rs:327-332).
+__device__ __forceinline__ void gather_sum(const InstDev& D, int n, double* f) {
 #pragma clang fp contract(off)
-    double f[3] = {0.0, 0.0, 0.0};
+    f[0] = f[1] = f[2] = 0.0;
     const int E = D.E;
     const int kb = D.n2e_ptr[n], ke = D.n2e_ptr[n + 1];
     // up to 8 incidences (every node of a hex mesh without merged nodes):
@@ -653,21 +677,38 @@ __device__ __forceinline__ bool gather_node(const InstDev& D, int n, double dt, 
         f[1] += (double)w.y;
         f[2] += (double)w.z;
     }
-    const double im = D.inv_mass[n];
-    const uint8_t fl = D.fixed[n];
-    const bool fx = (fl & NODE_FIXED) != 0;
-    const Double3 v = ld_d3(D.v, n);
-    const Double3 fe = (fl & NODE_CONTACT) ? ld_d3(D.f_ext, n) : Double3{0.0, 0.0, 0.0};
-    const double vd[3] = {v.x, v.y, v.z}, fed[3] = {fe.x, fe.y, fe.z};
-    double vn[3];
+}
+
+// Second half step (src/solver.rs:333-342): a = (f_ext - f_int) / m, v =
+// (v_1/2 + dt/2 a) damp, 0 at a fixed node.  Returns false when v is not
+// finite.
+__device__ __forceinline__ bool gather_update(const double* f, const double* fe, double im, double dt, double damp,
+                                              bool fixed, const double* v, double* vn, double* a) {
+#pragma clang fp contract(off)
     bool ok = true;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
-        const double a = (fed[d] - f[d]) * im;
-        vn[d] = (vd[d] + 0.5 * dt * a) * damp;
-        if (fx) vn[d] = 0.0;
+        a[d] = (fe[d] - f[d]) * im;
+        vn[d] = (v[d] + 0.5 * dt * a[d]) * damp;
+        if (fixed) vn[d] = 0.0;
         ok = ok && isfinite(vn[d]);
     }
+    return ok;
+}
+
+// Gather and second half step of node n with contact forces fext (f_int
+// and v written).  Returns false when the node's state is not finite.
+template <bool CHECK_U = true>
+__device__ __forceinline__ bool gather_node(const InstDev& D, int n, double dt, double damp, const double* fext) {
+    double f[3];
+    gather_sum(D, n, f);
+    const double im = D.inv_mass[n];
+    const uint8_t fl = D.fixed[n];
+    const Double3 v = ld_d3(D.v, n);
+    const Double3 fe = (fl & NODE_CONTACT) ? ld_d3(fext, n) : Double3{0.0, 0.0, 0.0};
+    const double vd[3] = {v.x, v.y, v.z}, fed[3] = {fe.x, fe.y, fe.z};
+    double vn[3], a[3];
+    bool ok = gather_update(f, fed, im, dt, damp, (fl & NODE_FIXED) != 0, vd, vn, a);
     st_d3(D.f_int, n, Double3{f[0], f[1], f[2]});
     st_d3(D.v, n, Double3{vn[0], vn[1], vn[2]});
     if (CHECK_U) {
@@ -943,39 +984,67 @@ __device__ void group_means(const InstDev& D, int g, double* s_w, double* out) {
     for (int k = 0; k < 7; ++k) out[k] = block_reduce<B>(s[k], 0, s_w);
 }
 
+// Local-x component of acceleration a (src/model.rs:45-61): a projected on
+// the unit vector from the frame origin's position xo to its x node's
+// position xx (a[0] when the frame is degenerate).
+__device__ __forceinline__ double local_x(const double* a, const double* xo, const double* xx) {
+#pragma clang fp contract(off)
+    double e[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) e[d] = xx[d] - xo[d];
+    const double en = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
+    if (en < 1e-300) return a[0];
+    return (e[0] / en) * a[0] + (e[1] / en) * a[1] + (e[2] / en) * a[2];
+}
+
 // Local-x acceleration of group g's accelerometer node after a step
 // (src/solver.rs:150-187 with the acceleration of :333-342): a = (f_ext -
 // f_int) / m at the node (0 for fixed nodes), projected on the unit vector
 // from the frame's origin node to its x node in the current configuration
-// (src/model.rs:45-61).
-__device__ __forceinline__ double accel_local_x(const InstDev& D, int g) {
+// (src/model.rs:45-61).  fext: the step's contact forces.
+__device__ __forceinline__ double accel_local_x(const InstDev& D, int g, const double* fext) {
 #pragma clang fp contract(off)
     const int n = D.g_accel[g];
     if (n < 0 || (D.fixed[n] & NODE_FIXED)) return 0.0;
     const double im = D.inv_mass[n];
     double a[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) a[d] = (D.f_ext[3 * (size_t)n + d] - D.f_int[3 * (size_t)n + d]) * im;
+    for (int d = 0; d < 3; ++d) a[d] = (fext[3 * (size_t)n + d] - D.f_int[3 * (size_t)n + d]) * im;
     const int o = D.g_frame[3 * g], x = D.g_frame[3 * g + 1];
     if (o < 0) return a[0];
-    double e[3];
+    double xo[3], xx[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d)
-        e[d] = (D.X[3 * (size_t)x + d] + D.u[3 * (size_t)x + d]) - (D.X[3 * (size_t)o + d] + D.u[3 * (size_t)o + d]);
-    const double en = sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]);
-    if (en < 1e-300) return a[0];
-    return (e[0] / en) * a[0] + (e[1] / en) * a[1] + (e[2] / en) * a[2];
+    for (int d = 0; d < 3; ++d) {
+        xo[d] = D.X[3 * (size_t)o + d] + D.u[3 * (size_t)o + d];
+        xx[d] = D.X[3 * (size_t)x + d] + D.u[3 * (size_t)x + d];
+    }
+    return local_x(a, xo, xx);
 }
 
-// Streams the sample (tn, a_g) of every group into the pulses: each grid
-// time k pulse_dt in (t_prev, tn] gets the linear interpolation between the
-// previous sample and this one (the resampling of src/signal.rs:43-64 done
-// as the samples arrive).  Thread 0 of the instance's block only.
-__device__ __forceinline__ void pulse_sample(const InstDev& D, StepCtl* c, double tn) {
+// Record slot of probe node n (batched path; -1 when n is not a probe).
+__device__ __forceinline__ int probe_slot(const InstDev& D, int n) {
+    for (int k = 0; k < D.n_probe; ++k)
+        if (D.probe_node[k] == n) return k;
+    return -1;
+}
+
+// accel_local_x from the probe records of the last gather (batched path).
+__device__ __forceinline__ double accel_local_x_probed(const InstDev& D, int g) {
+    const int sa = D.g_probe[3 * g], so = D.g_probe[3 * g + 1], sx = D.g_probe[3 * g + 2];
+    if (sa < 0) return 0.0;
+    const double* a = D.probe + 6 * (size_t)sa;
+    if (so < 0) return a[0];
+    return local_x(a, D.probe + 6 * (size_t)so + 3, D.probe + 6 * (size_t)sx + 3);
+}
+
+// Streams the sample (tn, a[g]) of every group g < 8 into the pulses: each
+// grid time k pulse_dt in (t_prev, tn] gets the linear interpolation
+// between the previous sample and this one (the resampling of
+// src/signal.rs:43-64 done as the samples arrive).  Thread 0 of the
+// instance's block only.
+__device__ __forceinline__ void pulse_sample(const InstDev& D, StepCtl* c, double tn, const double* a) {
 #pragma clang fp contract(off)
     const int ng = D.n_groups < 8 ? D.n_groups : 8;
-    double a[8];
-    for (int g = 0; g < ng; ++g) a[g] = accel_local_x(D, g);
     const double tp = c->pulse_t;
     int k = c->pulse_k;
     for (; k < D.n_pulse; ++k) {
@@ -1057,13 +1126,13 @@ __device__ __forceinline__ void node_position(const InstDev& D, int n, double* x
 }
 
 // Acceleration (f_ext - f_int) / m of node n after a step, 0 at fixed nodes
-// (src/solver.rs:333-342).
-__device__ __forceinline__ void node_accel(const InstDev& D, int n, double* a) {
+// (src/solver.rs:333-342); fext: the step's contact forces.
+__device__ __forceinline__ void node_accel(const InstDev& D, int n, double* a, const double* fext) {
 #pragma clang fp contract(off)
     const bool fx = (D.fixed[n] & NODE_FIXED) != 0;
     const double im = D.inv_mass[n];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) a[d] = fx ? 0.0 : (D.f_ext[3 * (size_t)n + d] - D.f_int[3 * (size_t)n + d]) * im;
+    for (int d = 0; d < 3; ++d) a[d] = fx ? 0.0 : (fext[3 * (size_t)n + d] - D.f_int[3 * (size_t)n + d]) * im;
 }
 
 // Unit axes (rows e1, e2, e3) of a body-fixed frame in the current
@@ -1111,9 +1180,10 @@ __device__ __forceinline__ void frame_axes(const InstDev& D, int o, int x, int p
 // mass, mass-weighted mean displacement, velocity and acceleration and
 // kinetic energy (block sums in a fixed order), and each accelerometer
 // node's position, displacement, velocity and acceleration with its
-// frame's origin and axes.  Every thread of the block calls it.
+// frame's origin and axes.  Every thread of the block calls it; fext: the
+// step's contact forces.
 template <int B>
-__device__ void history_sample(const InstDev& D, int k, long long step, double tn, double* s_w) {
+__device__ void history_sample(const InstDev& D, int k, long long step, double tn, double* s_w, const double* fext) {
 #pragma clang fp contract(off)
     const int na = D.n_groups < 8 ? D.n_groups : 8;
     double* rec = D.hist + (size_t)k * hist_record_len(D.n_parts, D.n_groups);
@@ -1125,7 +1195,7 @@ __device__ void history_sample(const InstDev& D, int k, long long step, double t
             const int n = D.p_nodes[i];
             const double m = D.mass[n];
             double a[3];
-            node_accel(D, n, a);
+            node_accel(D, n, a, fext);
             sm[0] += m;
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
@@ -1162,7 +1232,7 @@ __device__ void history_sample(const InstDev& D, int k, long long step, double t
                 r[3 + d] = D.u[3 * (size_t)n + d];
                 r[6 + d] = D.v[3 * (size_t)n + d];
             }
-            node_accel(D, n, r + 9);
+            node_accel(D, n, r + 9, fext);
             const int o = D.g_frame[3 * g], x = D.g_frame[3 * g + 1], pl = D.g_frame[3 * g + 2];
             if (o >= 0) {
                 node_position(D, o, r + 12);
@@ -1179,8 +1249,10 @@ __device__ void history_sample(const InstDev& D, int k, long long step, double t
 // decision (the loop condition of src/solver.rs:287 plus the optional
 // retirement at separation).  c is the block's control block in LDS; only
 // thread 0 writes it, and every thread returns the same stop decision.
+// fext: the step's contact forces.
 template <int B>
-__device__ bool step_end(const InstDev& D, StepCtl* c, double dt, int pair_hits, bool bad, double* s_w) {
+__device__ bool step_end(const InstDev& D, StepCtl* c, double dt, int pair_hits, bool bad, double* s_w,
+                         const double* fext) {
     const long long step = c->step + 1;
     double cr[8];
     const int ng = D.n_groups < 8 ? D.n_groups : 8;
@@ -1200,11 +1272,15 @@ __device__ bool step_end(const InstDev& D, StepCtl* c, double dt, int pair_hits,
     const bool done = sep || bad || !(tn < c->end_time);
     const int hk = c->hist_k;
     const bool hist = D.n_hist > 0 && (step % D.hist_steps == 0 || done);
-    if (hist && hk < D.n_hist) history_sample<B>(D, hk, step, tn, s_w);
+    if (hist && hk < D.n_hist) history_sample<B>(D, hk, step, tn, s_w, fext);
     __syncthreads();
     if (threadIdx.x == 0) {
         if (hist) c->hist_k = hk + 1;
-        if (D.n_pulse > 0) pulse_sample(D, c, tn);
+        if (D.n_pulse > 0) {
+            double a[8];
+            for (int g = 0; g < ng; ++g) a[g] = accel_local_x(D, g, fext);
+            pulse_sample(D, c, tn, a);
+        }
         for (int g = 0; g < ng; ++g) c->crush_max[g] = fmax(c->crush_max[g], cr[g]);
         c->had_pair = had_pair;
         c->quiet = quiet;

@@ -24,8 +24,10 @@ static_assert(CPK_VOL < CPK && CPK % 16 == 0 && CPK_H % 16 == 0, "class block la
 constexpr int SEARCH_BLOCK = 256;
 constexpr int FACE_HITS_CAP = 16;  // recorded hits per face before the scan fallback
 // Node flags (Prepared::fixed): a fixed node; a node contact forces reach
-// (its f_ext is written every step; every other node's stays 0).
-constexpr uint8_t NODE_FIXED = 1, NODE_CONTACT = 2;
+// (its f_ext is written every step; every other node's stays 0); on the
+// batched path, a node whose gathered acceleration or position the pulses
+// read (set in the instance image only).
+constexpr uint8_t NODE_FIXED = 1, NODE_CONTACT = 2, NODE_PROBE = 4;
 
 // Per-material parameters as the force kernels read them (f32).
 struct MatParams {

@@ -157,6 +157,7 @@ struct DeviceModel {
         I.v = v.p;
         I.f_int = f_int.p;
         I.f_ext = f_ext.p;
+        I.f_ext2 = f_ext.p;
         I.inv_mass = inv_mass.p;
         I.mass = mass.p;
         I.added = added.p;
@@ -295,13 +296,13 @@ __global__ __launch_bounds__(256) void k_contact_faceforce(InstDev D) {
 __global__ __launch_bounds__(256) void k_contact_apply(InstDev D) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= D.T || D.ctl->done) return;
-    contact_apply_node(D, i);
+    contact_apply_node(D, i, D.f_ext);
 }
 
 __global__ __launch_bounds__(256) void k_half(InstDev D) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= D.N || D.ctl->done) return;
-    half_node(D, n, D.ctl->dt);
+    half_node(D, n, D.ctl->dt, D.f_ext);
 }
 
 // Element forces of slots [begin, end) (one material kind), with per-lane
@@ -316,7 +317,7 @@ __global__ __launch_bounds__(256) void k_force(InstDev D, int begin, int end) {
 __global__ __launch_bounds__(256) void k_gather_update(InstDev D) {
     const int n = blockIdx.x * blockDim.x + threadIdx.x;
     if (n >= D.N || D.ctl->done) return;
-    if (!gather_node(D, n, D.ctl->dt, D.ctl->damping)) atomicOr(&D.ctl->error, 1);
+    if (!gather_node(D, n, D.ctl->dt, D.ctl->damping, D.f_ext)) atomicOr(&D.ctl->error, 1);
 }
 
 // End of step (one block).
@@ -326,7 +327,7 @@ __global__ __launch_bounds__(256) void k_step_end(InstDev D) {
     if (D.ctl->done) return;
     if (threadIdx.x == 0) c = *D.ctl;
     __syncthreads();
-    step_end<256>(D, &c, c.dt, c.pair_hits, c.error != 0, s_w);
+    step_end<256>(D, &c, c.dt, c.pair_hits, c.error != 0, s_w, D.f_ext);
     if (threadIdx.x == 0) *D.ctl = c;
 }
 

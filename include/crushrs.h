@@ -323,15 +323,17 @@ typedef struct crush_batch_stats {
     int64_t hc_element_steps;     /* honeycomb part of element_steps          */
     /* With kernel timing on (crush_batch_time_kernels), summed durations of
      * the kernels of the run, in ms: */
-    double k_end_ms;              /* per-instance end-of-step kernel          */
+    double k_end_ms;              /* per-instance control kernel (step end,
+                                     stable step)                            */
     double k_contact_ms;          /* contact: faces and grids, search, face
                                      reactions, apply                        */
-    double k_node_ms;             /* node half step and gather                */
+    double k_node_ms;             /* node pass: gather and next half step     */
     double k_force_hc_ms;         /* honeycomb element forces                 */
     double k_force_el_ms;         /* elastic element forces                   */
-    double k_ms[12];              /* each kernel of a step in launch order:
-                                     end, faces + grid, search, reactions,
-                                     apply, half, honeycomb, elastic, gather */
+    double k_ms[12];              /* each kernel of a tick in launch order:
+                                     control, faces + grid, search,
+                                     reactions, apply, node pass, honeycomb,
+                                     elastic (the rest 0)                    */
 } crush_batch_stats;
 
 /* Fill opt with the defaults. */
