@@ -357,6 +357,56 @@ __device__ __forceinline__ void dt_partial(const InstDev& D, double dt_floor, in
     }
 }
 
+// Crush of every group < 8 (group_crush's arithmetic, src/bin/crushrs.rs:
+// 265-271) in two parts, so that the loads of all groups are in flight
+// together: each thread's partial sums over the front nodes, the
+// accelerometer displacements and the front counts (0 where the crush is
+// 0); then, after crush_finish's block sums, the crush.
+struct CrushPart {
+    double s[8], ua[8];
+    int cnt[8];
+};
+
+template <int B>
+__device__ __forceinline__ void crush_partials(const InstDev& D, int ng, CrushPart& p) {
+#pragma clang fp contract(off)
+    const int t = threadIdx.x;
+    int b[8], e[8], ax[8], na[8], fr[8];
+#pragma unroll
+    for (int g = 0; g < 8; ++g)
+        if (g < ng) {
+            b[g] = D.g_front_ptr[g];
+            e[g] = D.g_front_ptr[g + 1];
+            ax[g] = D.g_axis[g];
+            na[g] = D.g_accel[g];
+        }
+#pragma unroll
+    for (int g = 0; g < 8; ++g)
+        if (g < ng) {
+            fr[g] = b[g] + t < e[g] ? D.g_front[b[g] + t] : -1;
+            p.ua[g] = na[g] >= 0 ? D.u[3 * (size_t)na[g] + ax[g]] : 0.0;
+            p.cnt[g] = (na[g] < 0 || e[g] <= b[g]) ? 0 : e[g] - b[g];
+        }
+#pragma unroll
+    for (int g = 0; g < 8; ++g)
+        if (g < ng) {
+            p.s[g] = 0.0;
+            if (fr[g] >= 0) p.s[g] += D.u[3 * (size_t)fr[g] + ax[g]];
+            for (int i = b[g] + t + B; i < e[g]; i += B) p.s[g] += D.u[3 * (size_t)D.g_front[i] + ax[g]];
+        }
+}
+
+template <int B>
+__device__ __forceinline__ void crush_finish(int ng, CrushPart& p, double* s_w, double* cr) {
+#pragma clang fp contract(off)
+#pragma unroll
+    for (int g = 0; g < 8; ++g)
+        if (g < ng) {
+            const double sum = block_reduce<B>(p.s[g], 0, s_w);
+            cr[g] = p.cnt[g] == 0 ? 0.0 : fabs(p.ua[g] - sum / (double)p.cnt[g]);
+        }
+}
+
 // Control pass of a tick (src/solver.rs:287-352), one workgroup per active
 // instance.  A tick runs, per instance, the contact, first half step and
 // element forces of one step (PH_HALF) and the gather of the step before it
@@ -397,6 +447,21 @@ __global__ __launch_bounds__(B) void k_ctl(const InstDev* __restrict__ table, co
     if (c->done) return;
     const int ph = c->phase;
     const int ng = D.n_groups < 8 ? D.n_groups : 8;
+    // loads that no decision below changes, issued together: the probe
+    // samples (a), the crush inputs of the step in flight (b) and the force
+    // tiles' stable-step partials (c), reduced per thread in slot order
+    if ((ph & PH_GATHER) && (ph & PH_HALF) && D.n_pulse > 0 && t < ng) s_a[t] = accel_local_x_probed(D, t);
+    CrushPart cp;
+    if (ph & PH_HALF) crush_partials<B>(D, ng, cp);
+    double mn = INFINITY, sm = 0.0;
+    if (c->adaptive) {
+        const int T = force_tile;
+        const int slots = (D.n_hc + T - 1) / T + (D.E - D.n_hc + T - 1) / T;
+        for (int j = t; j < slots; j += B) {
+            mn = fmin(mn, D.dtp_min[j]);
+            sm += D.dtp_sum[j];
+        }
+    }
     // (a) the last tick's gather
     if (ph & PH_GATHER) {
         bool stop;
@@ -406,9 +471,7 @@ __global__ __launch_bounds__(B) void k_ctl(const InstDev* __restrict__ table, co
                 const int hk = c->hist_k;
                 if (D.n_hist > 0 && hk < D.n_hist) history_sample<B>(D, hk, c->step, c->t, s_w, fext_step(D, c->step));
             }
-            // thread g forms group g's sample
-            if (D.n_pulse > 0 && t < ng) s_a[t] = accel_local_x_probed(D, t);
-            __syncthreads();
+            __syncthreads();  // thread g has formed group g's sample in s_a
             if (t == 0) {
                 if (D.n_pulse > 0) pulse_sample(D, c, c->t, s_a);
                 if (stop) {
@@ -466,7 +529,7 @@ __global__ __launch_bounds__(B) void k_ctl(const InstDev* __restrict__ table, co
             return;
         }
         double cr[8];
-        for (int g = 0; g < ng; ++g) cr[g] = group_crush<B>(D, g, s_w);
+        crush_finish<B>(ng, cp, s_w, cr);
         __syncthreads();
         if (t == 0) {
             for (int g = 0; g < ng; ++g) c->crush_max[g] = fmax(c->crush_max[g], cr[g]);
@@ -487,14 +550,7 @@ __global__ __launch_bounds__(B) void k_ctl(const InstDev* __restrict__ table, co
     // (src/solver.rs:288-299) from the force tiles' partials, reduced by the
     // whole block
     const bool check = c->adaptive && c->step % c->chk == 0;
-    double mn = INFINITY, sm = 0.0;
     if (check) {
-        const int T = force_tile;
-        const int slots = (D.n_hc + T - 1) / T + (D.E - D.n_hc + T - 1) / T;
-        for (int j = t; j < slots; j += B) {
-            mn = fmin(mn, D.dtp_min[j]);
-            sm += D.dtp_sum[j];
-        }
         mn = block_reduce<B>(mn, 1, s_w);
         sm = block_reduce<B>(sm, 0, s_w);
     }
