@@ -357,6 +357,18 @@ __device__ __forceinline__ void dt_partial(const InstDev& D, double dt_floor, in
     }
 }
 
+// Tile of workgroup b of a grid of `grid` workgroups.  Workgroups are
+// dispatched to the 8 XCDs in turn (workgroup b to XCD b mod 8).  Dealing
+// the tile list to the XCDs in chunks of `chunk` consecutive tiles keeps
+// neighbouring tiles of an instance, which share the nodes on their
+// boundaries, under one L2.  A grid that is not a multiple of 8 chunks
+// keeps the identity.
+__device__ __forceinline__ int xcd_tile(int b, int grid, int chunk) {
+    if (grid % (8 * chunk)) return b;
+    const int j = b >> 3;  // the workgroup's turn on its XCD
+    return ((j / chunk) * 8 + (b & 7)) * chunk + j % chunk;
+}
+
 // Crush of every group < 8 (group_crush's arithmetic, src/bin/crushrs.rs:
 // 265-271) in two parts, so that the loads of all groups are in flight
 // together: each thread's partial sums over the front nodes, the
@@ -706,9 +718,10 @@ __global__ __launch_bounds__(T) void k_apply_tiles(const InstDev* __restrict__ t
 // configuration (the positions and erosion the next step starts from).
 template <int KIND, int T>
 __device__ __forceinline__ void force_tile(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
-                                           const int32_t* __restrict__ n_tiles, double* s_w) {
-    if ((int)blockIdx.x >= *n_tiles) return;
-    const int2 tile = tiles[blockIdx.x];
+                                           const int32_t* __restrict__ n_tiles, int chunk, double* s_w) {
+    const int b = xcd_tile(blockIdx.x, gridDim.x, chunk);
+    if (b >= *n_tiles) return;
+    const int2 tile = tiles[b];
     const InstDev& D = table[tile.x];
     StepCtl& c = *D.ctl;
     if (!(c.phase & PH_HALF)) return;
@@ -729,18 +742,18 @@ __device__ __forceinline__ void force_tile(const InstDev* __restrict__ table, co
 
 template <int T>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(CRUSH_HC_WAVES, 8))) void k_force_hc(
-    const InstDev* __restrict__ table, const int2* __restrict__ tiles, const int32_t* __restrict__ n_tiles) {
+    const InstDev* __restrict__ table, const int2* __restrict__ tiles, const int32_t* __restrict__ n_tiles, int chunk) {
     __shared__ double s_w[T / 64];
-    force_tile<CRUSH_MAT_HONEYCOMB, T>(table, tiles, n_tiles, s_w);
+    force_tile<CRUSH_MAT_HONEYCOMB, T>(table, tiles, n_tiles, chunk, s_w);
 }
 
 // The elastic kernel at 4 waves per SIMD (128 VGPRs, some spills) runs 9 %
 // faster than at the 3 its 142 VGPRs would give.
 template <int T>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_force_el(
-    const InstDev* __restrict__ table, const int2* __restrict__ tiles, const int32_t* __restrict__ n_tiles) {
+    const InstDev* __restrict__ table, const int2* __restrict__ tiles, const int32_t* __restrict__ n_tiles, int chunk) {
     __shared__ double s_w[T / 64];
-    force_tile<CRUSH_MAT_ELASTIC, T>(table, tiles, n_tiles, s_w);
+    force_tile<CRUSH_MAT_ELASTIC, T>(table, tiles, n_tiles, chunk, s_w);
 }
 
 // Stable-step partials of instances that have not taken a step yet.
@@ -768,9 +781,10 @@ __global__ __launch_bounds__(T) void k_dt_tiles(const InstDev* __restrict__ tabl
 // the step after a gather alone) reads.
 template <int T>
 __global__ __launch_bounds__(T) void k_node_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
-                                                  const int32_t* __restrict__ n_tiles) {
-    if ((int)blockIdx.x >= *n_tiles) return;
-    const int2 tile = tiles[blockIdx.x];
+                                                  const int32_t* __restrict__ n_tiles, int chunk) {
+    const int b = xcd_tile(blockIdx.x, gridDim.x, chunk);
+    if (b >= *n_tiles) return;
+    const int2 tile = tiles[b];
     const InstDev& D = table[tile.x];
     StepCtl& c = *D.ctl;
     const int ph = c.phase;
@@ -967,6 +981,13 @@ void rebuild_lists(BatchImpl& bi) {
     for (int L = 0; L < N_LISTS; ++L) bi.cnt[L] = counts[L];
 }
 
+// Grid of a launch that deals its tiles to XCDs in chunks (xcd_tile): the
+// listed tiles rounded up to a multiple of 8 chunks.
+static unsigned grid8(int n, int chunk) {
+    const int m = 8 * chunk;
+    return (unsigned)((n + m - 1) / m * m);
+}
+
 template <int B, int T>
 static void launch_step(BatchImpl& bi, double cs, hipEvent_t* ev) {
     const int32_t* cnt = bi.d_counts.p;
@@ -1013,16 +1034,16 @@ static void launch_step(BatchImpl& bi, double cs, hipEvent_t* ev) {
                            bi.d_tiles[L_APPLY].p, cnt + L_APPLY);
     mark(K_NODE);
     if (bi.cap[L_NODE] > 0)
-        hipLaunchKernelGGL(k_node_tiles<NODE_TILE>, dim3(bi.cap[L_NODE]), dim3(NODE_TILE), 0, st, tab,
-                           bi.d_tiles[L_NODE].p, cnt + L_NODE);
+        hipLaunchKernelGGL(k_node_tiles<NODE_TILE>, dim3(grid8(bi.cap[L_NODE], bi.chunk[L_NODE])), dim3(NODE_TILE), 0, st, tab,
+                           bi.d_tiles[L_NODE].p, cnt + L_NODE, bi.chunk[L_NODE]);
     mark(K_FORCE_HC);
     if (bi.cap[L_HC] > 0)
-        hipLaunchKernelGGL(k_force_hc<T>, dim3(bi.cap[L_HC]), dim3(T), 0, st, tab,
-                           bi.d_tiles[L_HC].p, cnt + L_HC);
+        hipLaunchKernelGGL(k_force_hc<T>, dim3(grid8(bi.cap[L_HC], bi.chunk[L_HC])), dim3(T), 0, st, tab,
+                           bi.d_tiles[L_HC].p, cnt + L_HC, bi.chunk[L_HC]);
     mark(K_FORCE_EL);
     if (bi.cap[L_EL] > 0)
-        hipLaunchKernelGGL(k_force_el<T>, dim3(bi.cap[L_EL]), dim3(T), 0, st, tab,
-                           bi.d_tiles[L_EL].p, cnt + L_EL);
+        hipLaunchKernelGGL(k_force_el<T>, dim3(grid8(bi.cap[L_EL], bi.chunk[L_EL])), dim3(T), 0, st, tab,
+                           bi.d_tiles[L_EL].p, cnt + L_EL, bi.chunk[L_EL]);
     mark(N_KERNELS);
 }
 
@@ -1100,6 +1121,13 @@ void capture(BatchImpl& bi) {
         bi.graph = nullptr;
     }
     for (int L = 0; L < N_LISTS; ++L) bi.cap[L] = (int)((double)bi.cnt[L] * bi.headroom) + (bi.headroom > 1.0 ? 1 : 0);
+    // XCD chunks of the force and node tiles: an eighth of an instance's
+    // tiles (one instance spreads over the 8 L2s in 8 runs), at least 4
+    // (many instances: a measured 2 % faster honeycomb pass than 1)
+    for (int L : {L_HC, L_EL, L_NODE}) {
+        const int per_inst = (bi.cap[L] + std::max(bi.cap[L_INST], 1) - 1) / std::max(bi.cap[L_INST], 1);
+        bi.chunk[L] = std::max(4, (per_inst + 7) / 8);
+    }
     bi.need_capture = false;
     const int R = steps_per_graph(bi.opt);
     hipGraph_t g;

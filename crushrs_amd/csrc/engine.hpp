@@ -63,6 +63,7 @@ struct BatchImpl {
     std::vector<uint8_t> host_done;
     int cnt[N_LISTS] = {};  // entries in the current lists
     int cap[N_LISTS] = {};  // grids of the captured graph
+    int chunk[N_LISTS] = {1, 1, 1, 1, 1, 1, 1, 1, 1};  // tiles per XCD chunk (force and node lists)
     double headroom = 1.0;  // captured grids / current lists (server: > 1)
     bool need_capture = false;
     int grid_block = 64;    // k_grid workgroup: the largest moving contact's faces, in waves
