@@ -644,9 +644,10 @@ __global__ __launch_bounds__(B) void k_grid(const InstDev* __restrict__ table, c
 // the grid stale for k_grid.
 template <int T>
 __global__ __launch_bounds__(T) void k_facegeo_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
-                                                     const int32_t* __restrict__ n_tiles) {
-    if ((int)blockIdx.x >= *n_tiles) return;
-    const int2 tile = tiles[blockIdx.x];
+                                                     const int32_t* __restrict__ n_tiles, int chunk) {
+    const int b = xcd_tile(blockIdx.x, gridDim.x, chunk);
+    if (b >= *n_tiles) return;
+    const int2 tile = tiles[b];
     const InstDev& D = table[tile.x];
     if (!(D.ctl->phase & PH_HALF)) return;
     const int ci = tile.y >> 20;
@@ -666,9 +667,10 @@ __global__ __launch_bounds__(T) void k_facegeo_tiles(const InstDev* __restrict__
 template <int T, int L>
 __global__ __launch_bounds__(T * L) void k_search_tiles(const InstDev* __restrict__ table,
                                                         const int2* __restrict__ tiles,
-                                                        const int32_t* __restrict__ n_tiles) {
-    if ((int)blockIdx.x >= *n_tiles) return;
-    const int2 tile = tiles[blockIdx.x];
+                                                        const int32_t* __restrict__ n_tiles, int chunk) {
+    const int b = xcd_tile(blockIdx.x, gridDim.x, chunk);
+    if (b >= *n_tiles) return;
+    const int2 tile = tiles[b];
     const InstDev& D = table[tile.x];
     StepCtl& c = *D.ctl;
     if (!(c.phase & PH_HALF)) return;
@@ -691,9 +693,10 @@ __global__ __launch_bounds__(T * L) void k_search_tiles(const InstDev* __restric
 template <int T>
 __global__ __launch_bounds__(T) void k_faceforce_tiles(const InstDev* __restrict__ table,
                                                        const int2* __restrict__ tiles,
-                                                       const int32_t* __restrict__ n_tiles) {
-    if ((int)blockIdx.x >= *n_tiles) return;
-    const int2 tile = tiles[blockIdx.x];
+                                                       const int32_t* __restrict__ n_tiles, int chunk) {
+    const int b = xcd_tile(blockIdx.x, gridDim.x, chunk);
+    if (b >= *n_tiles) return;
+    const int2 tile = tiles[b];
     const InstDev& D = table[tile.x];
     if (!(D.ctl->phase & PH_HALF)) return;
     const int f = tile.y + (int)threadIdx.x;
@@ -702,9 +705,10 @@ __global__ __launch_bounds__(T) void k_faceforce_tiles(const InstDev* __restrict
 
 template <int T>
 __global__ __launch_bounds__(T) void k_apply_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
-                                                   const int32_t* __restrict__ n_tiles) {
-    if ((int)blockIdx.x >= *n_tiles) return;
-    const int2 tile = tiles[blockIdx.x];
+                                                   const int32_t* __restrict__ n_tiles, int chunk) {
+    const int b = xcd_tile(blockIdx.x, gridDim.x, chunk);
+    if (b >= *n_tiles) return;
+    const int2 tile = tiles[b];
     const InstDev& D = table[tile.x];
     const StepCtl& c = *D.ctl;
     if (!(c.phase & PH_HALF)) return;
@@ -1001,8 +1005,8 @@ static void launch_step(BatchImpl& bi, double cs, hipEvent_t* ev) {
                        T, bi.done_flag.p, bi.n_done.p);
     mark(K_GRID);
     if (bi.cap[L_FGEO] > 0)
-        hipLaunchKernelGGL(k_facegeo_tiles<CONTACT_TILE>, dim3(bi.cap[L_FGEO]), dim3(CONTACT_TILE), 0, st, tab,
-                           bi.d_tiles[L_FGEO].p, cnt + L_FGEO);
+        hipLaunchKernelGGL(k_facegeo_tiles<CONTACT_TILE>, dim3(grid8(bi.cap[L_FGEO], bi.chunk[L_FGEO])), dim3(CONTACT_TILE), 0, st, tab,
+                           bi.d_tiles[L_FGEO].p, cnt + L_FGEO, bi.chunk[L_FGEO]);
     if (bi.cap[L_GRID] > 0) {
         const dim3 g(bi.cap[L_GRID]);
         if (bi.grid_block == 64)
@@ -1019,19 +1023,19 @@ static void launch_step(BatchImpl& bi, double cs, hipEvent_t* ev) {
     mark(K_SEARCH);
     if (bi.cap[L_SEARCH] > 0)
         if (bi.search_lanes == 4)
-            hipLaunchKernelGGL((k_search_tiles<CONTACT_TILE, 4>), dim3(bi.cap[L_SEARCH]), dim3(4 * CONTACT_TILE), 0, st,
-                               tab, bi.d_tiles[L_SEARCH].p, cnt + L_SEARCH);
+            hipLaunchKernelGGL((k_search_tiles<CONTACT_TILE, 4>), dim3(grid8(bi.cap[L_SEARCH], bi.chunk[L_SEARCH])), dim3(4 * CONTACT_TILE), 0, st,
+                               tab, bi.d_tiles[L_SEARCH].p, cnt + L_SEARCH, bi.chunk[L_SEARCH]);
         else
-            hipLaunchKernelGGL((k_search_tiles<CONTACT_TILE, 1>), dim3(bi.cap[L_SEARCH]), dim3(CONTACT_TILE), 0, st,
-                               tab, bi.d_tiles[L_SEARCH].p, cnt + L_SEARCH);
+            hipLaunchKernelGGL((k_search_tiles<CONTACT_TILE, 1>), dim3(grid8(bi.cap[L_SEARCH], bi.chunk[L_SEARCH])), dim3(CONTACT_TILE), 0, st,
+                               tab, bi.d_tiles[L_SEARCH].p, cnt + L_SEARCH, bi.chunk[L_SEARCH]);
     mark(K_FACEFORCE);
     if (bi.cap[L_FACE] > 0)
-        hipLaunchKernelGGL(k_faceforce_tiles<CONTACT_TILE>, dim3(bi.cap[L_FACE]), dim3(CONTACT_TILE), 0, st, tab,
-                           bi.d_tiles[L_FACE].p, cnt + L_FACE);
+        hipLaunchKernelGGL(k_faceforce_tiles<CONTACT_TILE>, dim3(grid8(bi.cap[L_FACE], bi.chunk[L_FACE])), dim3(CONTACT_TILE), 0, st, tab,
+                           bi.d_tiles[L_FACE].p, cnt + L_FACE, bi.chunk[L_FACE]);
     mark(K_APPLY);
     if (bi.cap[L_APPLY] > 0)
-        hipLaunchKernelGGL(k_apply_tiles<CONTACT_TILE>, dim3(bi.cap[L_APPLY]), dim3(CONTACT_TILE), 0, st, tab,
-                           bi.d_tiles[L_APPLY].p, cnt + L_APPLY);
+        hipLaunchKernelGGL(k_apply_tiles<CONTACT_TILE>, dim3(grid8(bi.cap[L_APPLY], bi.chunk[L_APPLY])), dim3(CONTACT_TILE), 0, st, tab,
+                           bi.d_tiles[L_APPLY].p, cnt + L_APPLY, bi.chunk[L_APPLY]);
     mark(K_NODE);
     if (bi.cap[L_NODE] > 0)
         hipLaunchKernelGGL(k_node_tiles<NODE_TILE>, dim3(grid8(bi.cap[L_NODE], bi.chunk[L_NODE])), dim3(NODE_TILE), 0, st, tab,
@@ -1124,7 +1128,7 @@ void capture(BatchImpl& bi) {
     // XCD chunks of the force and node tiles: an eighth of an instance's
     // tiles (one instance spreads over the 8 L2s in 8 runs), at least 4
     // (many instances: a measured 2 % faster honeycomb pass than 1)
-    for (int L : {L_HC, L_EL, L_NODE}) {
+    for (int L : {L_HC, L_EL, L_NODE, L_FGEO, L_SEARCH, L_FACE, L_APPLY}) {
         const int per_inst = (bi.cap[L] + std::max(bi.cap[L_INST], 1) - 1) / std::max(bi.cap[L_INST], 1);
         bi.chunk[L] = std::max(4, (per_inst + 7) / 8);
     }
