@@ -700,7 +700,7 @@ __global__ __launch_bounds__(T) void k_faceforce_tiles(const InstDev* __restrict
     const InstDev& D = table[tile.x];
     if (!(D.ctl->phase & PH_HALF)) return;
     const int f = tile.y + (int)threadIdx.x;
-    if (f < D.F) contact_face_force(D, f);
+    contact_face_force_wave(D, f, f < D.F);
 }
 
 template <int T>
@@ -944,17 +944,19 @@ void rebuild_lists(BatchImpl& bi) {
     }
     // one k_grid thread per face of the largest contact whose faces move,
     // from one wave up to sixteen (larger contacts loop); a server (headroom
-    // above 1) only grows it, so arrivals and retirements do not force
-    // captures back and forth
+    // above 1) shrinks it only to a quarter or less, so arrivals and
+    // retirements do not force captures back and forth
     int gb = 64;
     while (gb < 1024 && gb < max_faces) gb *= 2;
-    if (gb > bi.grid_block || (gb < bi.grid_block && bi.headroom <= 1.0)) {
+    if (gb > bi.grid_block || (gb < bi.grid_block && (bi.headroom <= 1.0 || 4 * gb <= bi.grid_block))) {
         bi.grid_block = gb;
         bi.need_capture = true;
     }
-    // a batch with few search tiles gives each contacting node 4 lanes (the
-    // server keeps one: its lists change with every arrival)
-    const int sl = (bi.headroom <= 1.0 && (int)tiles[L_SEARCH].size() < SEARCH_SPLIT_TILES) ? 4 : 1;
+    // few search tiles: 4 lanes per contacting node; a server switches at
+    // half the threshold going down and at the threshold going up
+    const int ns = (int)tiles[L_SEARCH].size();
+    int sl = ns < SEARCH_SPLIT_TILES ? 4 : 1;
+    if (bi.headroom > 1.0 && ns >= SEARCH_SPLIT_TILES / 2 && ns < SEARCH_SPLIT_TILES) sl = bi.search_lanes;
     if (sl != bi.search_lanes) {
         bi.search_lanes = sl;
         bi.need_capture = true;

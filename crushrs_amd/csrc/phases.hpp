@@ -379,6 +379,47 @@ __device__ __forceinline__ void contact_face_force(const InstDev& D, int f) {
     D.fh_count[f] = 0;
 }
 
+// contact_face_force for the faces of one wave's lanes (f, where valid),
+// except that a face whose hits overflowed its list is summed by the whole
+// wave: 64 contacting nodes at a time, each hit added in ascending node
+// order as face_reaction's scan adds it, so the sums are bitwise the same.
+// Every lane of the wave calls it.
+__device__ __forceinline__ void contact_face_force_wave(const InstDev& D, int f, bool valid) {
+#pragma clang fp contract(off)
+    const int lane = threadIdx.x & 63;
+    const bool over = valid && D.fh_count[f] > FACE_HITS_CAP;
+    if (valid && !over) contact_face_force(D, f);
+    unsigned long long m = __ballot(over);
+    while (m) {
+        const int src = __builtin_ctzll(m);
+        m &= m - 1;
+        const int ff = __shfl(f, src, 64);
+        const ContactDesc cd = D.cdesc[D.cf_contact[ff]];
+        const int end = cd.node_begin + cd.n_nodes;
+        double s[3] = {0.0, 0.0, 0.0};
+        for (int base = cd.node_begin; base < end; base += 64) {
+            const int k = base + lane;
+            const bool hit = k < end && D.rec_face[k] == ff;
+            double r[3] = {0.0, 0.0, 0.0};
+            if (hit)
+#pragma unroll
+                for (int d = 0; d < 3; ++d) r[d] = D.rec_f[3 * (size_t)k + d];
+            unsigned long long h = __ballot(hit);
+            while (h) {
+                const int j = __builtin_ctzll(h);
+                h &= h - 1;
+#pragma unroll
+                for (int d = 0; d < 3; ++d) s[d] += 0.25 * __shfl(r[d], j, 64);
+            }
+        }
+        if (lane == src) {
+#pragma unroll
+            for (int d = 0; d < 3; ++d) D.face_force[3 * (size_t)ff + d] = s[d];
+            D.fh_count[ff] = 0;
+        }
+    }
+}
+
 // External (contact) force of touched node i, in contact order, into fext.
 // With FUSED the reactions of the node's faces are formed here (the same
 // arithmetic as contact_face_force) instead of read from face_force.

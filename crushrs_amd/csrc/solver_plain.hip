@@ -289,8 +289,8 @@ __global__ __launch_bounds__(256) void k_contact_search(InstDev D, double cell_s
 
 __global__ __launch_bounds__(256) void k_contact_faceforce(InstDev D) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= D.F || D.ctl->done) return;
-    contact_face_force(D, f);
+    if (D.ctl->done) return;
+    contact_face_force_wave(D, f, f < D.F);
 }
 
 __global__ __launch_bounds__(256) void k_contact_apply(InstDev D) {
