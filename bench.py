@@ -8,7 +8,8 @@ Workloads
   sweep  (default) the batched sweep: N (4096) independent head-on/offset
          pairs of the four tuned vehicle TOMLs, speeds U(10, 40) mph each,
          lateral offset U(0, 0.6) m, element size 0.3 or 0.2 m, rigid road,
-         0.15 s each, solved together by the batched persistent kernel.
+         0.15 s each, solved together by the batched engine (graph-replayed
+         tile kernels).
          One bench step is one batched solve of every instance to its end.
   calib  the batched calibration set: N (1024) rigid-barrier tests at
          35 mph of the four TOMLs at 0.3 m, crush curves perturbed as a
@@ -339,7 +340,12 @@ def bench_sweep(args):
     progress(leg="build", models=len(models), s=round(t_build, 2))
     opt = ca.batch_options(threads_per_instance=args.threads_per_instance, force_tile=args.force_tile,
                            steps_per_graph=args.steps_per_graph, autotune=int(args.autotune))
+    s0 = time.perf_counter()
     batch = ca.Batch(models, opt)
+    if args.autotune:
+        progress(leg="autotune", s=round(time.perf_counter() - s0, 2),
+                 threads_per_instance=batch.options.threads_per_instance, force_tile=batch.options.force_tile,
+                 class_scales=batch.class_scales())
     if args.eager:
         batch.time_kernels(True)
     runs = timed_runs(batch, args.steps, args.warmup, args.budget_s, "batch")
@@ -394,6 +400,8 @@ def bench_sweep(args):
             "solver_steps": {"min": int(steps.min()), "max": int(steps.max()), "mean": float(steps.mean())},
             "threads_per_instance": batch.options.threads_per_instance, "force_tile": batch.options.force_tile,
             "contact_cell_scale": batch.options.contact_cell_scale, "steps_per_graph": batch.options.steps_per_graph,
+            "autotune": ({"class_scales": [{"max_elements": e, "cell_scale": cs} for e, cs in batch.class_scales()]}
+                         if args.autotune else None),
         },
         "step_definition": (f"one run of all {n} instances to their end; steps requested {args.steps}, warm-up "
                             f"requested {args.warmup}; runs are timed until {args.steps} are done or the next would "
@@ -765,7 +773,9 @@ def main():
     ap.add_argument("--threads-per-instance", type=int, default=256)
     ap.add_argument("--force-tile", type=int, default=256)
     ap.add_argument("--steps-per-graph", type=int, default=None)
-    ap.add_argument("--autotune", action="store_true")
+    ap.add_argument("--autotune", action=argparse.BooleanOptionalAction, default=None,
+                    help="time the launch shapes and per-size-class grid scales before the run (default: on for "
+                         "sweep and calib, off with --eager, whose profiler runs cannot follow graph launches)")
     ap.add_argument("--budget-s", type=float, default=90.0,
                     help="batched workloads: timed runs stop once the next would pass this much timed wall time")
     ap.add_argument("--end-time", type=float, default=0.15,
@@ -778,6 +788,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
+    if args.autotune is None:
+        args.autotune = args.workload in ("sweep", "calib") and not args.eager
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; independent instances would shard across GPUs "
                                    "with no communication, but only --gpus 1 is built and measured"}))

@@ -410,6 +410,14 @@ class Batch:
         check(self._lib.crush_batch_options_get(self._h, C.byref(o)))
         return o
 
+    def class_scales(self) -> list[tuple[int, float]]:
+        """Autotuned contact grid cell scale per size class: (largest element
+        count of the class, scale), classes ascending; empty when the batch
+        was not autotuned."""
+        scale, elems, k = (C.c_double * 16)(), (C.c_int32 * 16)(), C.c_int32()
+        check(self._lib.crush_batch_class_scales(self._h, 16, scale, elems, C.byref(k)))
+        return [(int(elems[i]), float(scale[i])) for i in range(min(k.value, 16))]
+
     def result(self, i: int, fields: bool = True) -> Results:
         mdl = self.models[i]
         hc = mdl.output_capacity()[0] if mdl.settings.history_steps > 0 else 0

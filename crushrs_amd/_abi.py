@@ -302,6 +302,8 @@ SIGNATURES = {
     "crush_batch_reset": (C.c_int32, [C.c_void_p]),
     "crush_batch_options_get": (C.c_int32, [C.c_void_p, C.POINTER(BatchOptions)]),
     "crush_batch_time_kernels": (C.c_int32, [C.c_void_p, C.c_int32]),
+    "crush_batch_class_scales": (C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(C.c_double), C.POINTER(C.c_int32),
+                                             C.POINTER(C.c_int32)]),
     "crush_batch_result": (C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(Results)]),
     "crush_batch_free": (None, [C.c_void_p]),
     "crush_server_defaults": (None, [C.POINTER(ServerOptions)]),

@@ -310,6 +310,7 @@ InstDev make_inst(const Prepared& P, const ImageOffsets& o, uintptr_t base) {
     I.probe = at<double>(base, o.probe);
     I.n_probe = (int)probe_nodes(P).size();
     I.g_probe = at<int32_t>(base, o.g_probe);
+    I.cell_scale = 1.0;
     return I;
 }
 
@@ -594,7 +595,7 @@ __global__ __launch_bounds__(B) void k_ctl(const InstDev* __restrict__ table, co
 // accepting face index.
 template <int B>
 __global__ __launch_bounds__(B) void k_grid(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
-                                            const int32_t* __restrict__ n_tiles, double cell_scale) {
+                                            const int32_t* __restrict__ n_tiles) {
     __shared__ double s_w[B / 64];
     __shared__ int s_tmp[B / 64];
     if ((int)blockIdx.x >= *n_tiles) return;
@@ -626,12 +627,12 @@ __global__ __launch_bounds__(B) void k_grid(const InstDev* __restrict__ table, c
         // starts copied out once
         __shared__ int l_start[LDS_GRID_CELLS + 1];
         __shared__ int l_cursor[LDS_GRID_CELLS];
-        g = build_face_grid<B>(D, cd, cell_scale, min(cd.grid_cells, LDS_GRID_CELLS), cd.grid_entries, l_start,
+        g = build_face_grid<B>(D, cd, D.cell_scale, min(cd.grid_cells, LDS_GRID_CELLS), cd.grid_entries, l_start,
                                l_cursor, ge, s_w, s_tmp, GRID_MARGIN);
         const int nc = g.overflow ? 1 : g.dim[0] * g.dim[1] * g.dim[2];
         for (int i = threadIdx.x; i <= nc; i += B) gs[i] = l_start[i];
     } else {
-        g = build_face_grid<B>(D, cd, cell_scale, cd.grid_cells, cd.grid_entries, gs, ge + cd.grid_entries, ge, s_w,
+        g = build_face_grid<B>(D, cd, D.cell_scale, cd.grid_cells, cd.grid_entries, gs, ge + cd.grid_entries, ge, s_w,
                                s_tmp, GRID_MARGIN);
     }
     if (threadIdx.x == 0) D.cgp[ci] = g;
@@ -907,6 +908,8 @@ static int force_tile(const crush_batch_options& o) {
 
 int steps_per_graph(const crush_batch_options& o) { return o.steps_per_graph > 0 ? o.steps_per_graph : 32; }
 
+double cell_scale_of(const crush_batch_options& o) { return o.contact_cell_scale > 0.0 ? o.contact_cell_scale : 1.0; }
+
 
 // Tiles of instance i for list L with force tile T.
 static void add_tiles(const Prepared& P, int i, int T, std::vector<int2>* out) {
@@ -995,7 +998,7 @@ static unsigned grid8(int n, int chunk) {
 }
 
 template <int B, int T>
-static void launch_step(BatchImpl& bi, double cs, hipEvent_t* ev) {
+static void launch_step(BatchImpl& bi, hipEvent_t* ev) {
     const int32_t* cnt = bi.d_counts.p;
     const InstDev* tab = bi.table.p;
     hipStream_t st = bi.stream;
@@ -1012,15 +1015,15 @@ static void launch_step(BatchImpl& bi, double cs, hipEvent_t* ev) {
     if (bi.cap[L_GRID] > 0) {
         const dim3 g(bi.cap[L_GRID]);
         if (bi.grid_block == 64)
-            hipLaunchKernelGGL(k_grid<64>, g, dim3(64), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID, cs);
+            hipLaunchKernelGGL(k_grid<64>, g, dim3(64), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID);
         else if (bi.grid_block == 128)
-            hipLaunchKernelGGL(k_grid<128>, g, dim3(128), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID, cs);
+            hipLaunchKernelGGL(k_grid<128>, g, dim3(128), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID);
         else if (bi.grid_block == 256)
-            hipLaunchKernelGGL(k_grid<256>, g, dim3(256), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID, cs);
+            hipLaunchKernelGGL(k_grid<256>, g, dim3(256), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID);
         else if (bi.grid_block == 512)
-            hipLaunchKernelGGL(k_grid<512>, g, dim3(512), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID, cs);
+            hipLaunchKernelGGL(k_grid<512>, g, dim3(512), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID);
         else
-            hipLaunchKernelGGL(k_grid<1024>, g, dim3(1024), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID, cs);
+            hipLaunchKernelGGL(k_grid<1024>, g, dim3(1024), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID);
     }
     mark(K_SEARCH);
     if (bi.cap[L_SEARCH] > 0)
@@ -1064,14 +1067,14 @@ static void launch_dt_init(BatchImpl& bi) {
 }
 
 template <int B>
-static void launch_step_t(BatchImpl& bi, double cs, hipEvent_t* ev) {
+static void launch_step_t(BatchImpl& bi, hipEvent_t* ev) {
     const int T = force_tile(bi.opt);
     if (T == 64)
-        launch_step<B, 64>(bi, cs, ev);
+        launch_step<B, 64>(bi, ev);
     else if (T == 128)
-        launch_step<B, 128>(bi, cs, ev);
+        launch_step<B, 128>(bi, ev);
     else
-        launch_step<B, 256>(bi, cs, ev);
+        launch_step<B, 256>(bi, ev);
 }
 
 constexpr int EV_PER_STEP = N_KERNELS + 1;
@@ -1097,16 +1100,15 @@ void collect(BatchImpl& bi, int n) {
 
 // One step's launches; ev (EV_PER_STEP events, or null) brackets the kernels.
 static void launch_one(BatchImpl& bi, hipEvent_t* ev) {
-    const double cs = bi.opt.contact_cell_scale > 0.0 ? bi.opt.contact_cell_scale : 1.0;
     const int B = inst_block(bi.opt);
     if (B == 64)
-        launch_step_t<64>(bi, cs, ev);
+        launch_step_t<64>(bi, ev);
     else if (B == 128)
-        launch_step_t<128>(bi, cs, ev);
+        launch_step_t<128>(bi, ev);
     else if (B == 512)
-        launch_step_t<512>(bi, cs, ev);
+        launch_step_t<512>(bi, ev);
     else
-        launch_step_t<256>(bi, cs, ev);
+        launch_step_t<256>(bi, ev);
 }
 
 // Add the kernel durations of the last replay.
@@ -1184,12 +1186,13 @@ static void upload_images(BatchImpl& bi) {
     }
 }
 
-// Restore the initial state and the full active lists.
-static void reset_state(BatchImpl& bi, bool recapture) {
+// Restore the initial state and the full active lists (with `only`, the
+// lists of the instances it marks).
+static void reset_state(BatchImpl& bi, bool recapture, const std::vector<uint8_t>* only = nullptr) {
     upload_images(bi);
     bi.n_done.zero(bi.stream);
     bi.done_flag.zero(bi.stream);
-    std::fill(bi.host_done.begin(), bi.host_done.end(), 0);
+    for (size_t i = 0; i < bi.host_done.size(); ++i) bi.host_done[i] = only ? !(*only)[i] : 0;
     rebuild_lists(bi);
     bool same = !bi.need_capture;
     for (int L = 0; L < N_LISTS; ++L) same = same && bi.cap[L] == bi.cnt[L];
@@ -1248,10 +1251,12 @@ BatchImpl* batch_create(const crush_model* const* models, int n, const crush_bat
     }
     HIP_OK(hipStreamCreateWithFlags(&bi->stream, hipStreamNonBlocking));
     bi->pool.alloc(total);
-    std::vector<InstDev> tab(n);
+    std::vector<InstDev>& tab = bi->host_tab;
+    tab.resize(n);
     const uintptr_t base = reinterpret_cast<uintptr_t>(bi->pool.p);
     for (int i = 0; i < n; ++i) {
         tab[i] = make_inst(bi->prep[i], bi->offs[i], base + bi->base_off[i]);
+        tab[i].cell_scale = cell_scale_of(bi->opt);
         HIP_OK(hipMemcpyAsync(bi->pool.p + bi->base_off[i], images[i].data(), images[i].size(), hipMemcpyHostToDevice,
                               bi->stream));
     }
@@ -1376,42 +1381,54 @@ void batch_run(BatchImpl& bi, crush_batch_stats* st) {
                  bi.host_summary[i].steps);
 }
 
-// Timed trials over the knobs that do not change results (workgroup size
-// of the per-instance kernel, force tile, contact grid cell scale): each
-// candidate runs a few graph replays from the initial state; the fastest
-// is kept and the state restored.
+// Copy the host table into the device table (same buffer: captured
+// graphs stay valid).
+static void put_table(BatchImpl& bi) {
+    HIP_OK(hipMemcpyAsync(bi.table.p, bi.host_tab.data(), bi.host_tab.size() * sizeof(InstDev), hipMemcpyHostToDevice,
+                          bi.stream));
+    HIP_OK(hipStreamSynchronize(bi.stream));
+}
+
+// Milliseconds of `replays` graph replays from the initial state of the
+// instances `only` marks (all when null), after one replay to warm up.
+static double trial_ms(BatchImpl& bi, const std::vector<uint8_t>* only, long long replays) {
+    reset_state(bi, true, only);
+    bi.replays = 0;
+    drive(bi, 1);
+    hipEvent_t e0, e1;
+    HIP_OK(hipEventCreate(&e0));
+    HIP_OK(hipEventCreate(&e1));
+    HIP_OK(hipEventRecord(e0, bi.stream));
+    bi.replays = 0;
+    drive(bi, replays);
+    HIP_OK(hipEventRecord(e1, bi.stream));
+    HIP_OK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIP_OK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return ms;
+}
+
+// Timed trials over the knobs that do not change results (the reference
+// fixes its own, src/kernel_simd.rs:562).  (1) The launch shapes, over the
+// whole batch: workgroup size of the per-instance kernel and force tile.
+// (2) Per size class (instances in up to four classes of equal count by
+// element count), the contact grids' cell scale, each class timed alone.
+// Each candidate runs a few graph replays from the initial state; the
+// fastest is kept and the state restored.
 static void autotune(BatchImpl& bi) {
+    const int n = (int)bi.prep.size();
     const crush_batch_options base = bi.opt;
-    struct Cand {
-        int b, t;
-        double cs;
-    };
-    const Cand cands[] = {{256, 256, 1.0}, {128, 256, 1.0}, {512, 256, 1.0}, {256, 128, 1.0},
-                          {256, 64, 1.0},  {256, 256, 0.5}, {256, 256, 2.0}};
+    const long long trial_replays = 3;
+    const int shapes[][2] = {{256, 256}, {128, 256}, {512, 256}, {256, 128}, {256, 64}};
     double best = INFINITY;
     crush_batch_options best_opt = base;
-    const long long trial_replays = 3;
-    for (const Cand& cd : cands) {
+    for (const auto& sh : shapes) {
         bi.opt = base;
-        bi.opt.threads_per_instance = cd.b;
-        bi.opt.force_tile = cd.t;
-        bi.opt.contact_cell_scale = cd.cs;
-        reset_state(bi, true);
-        // one replay to warm up, then time the rest
-        bi.replays = 0;
-        drive(bi, 1);
-        hipEvent_t e0, e1;
-        HIP_OK(hipEventCreate(&e0));
-        HIP_OK(hipEventCreate(&e1));
-        HIP_OK(hipEventRecord(e0, bi.stream));
-        bi.replays = 0;
-        drive(bi, trial_replays);
-        HIP_OK(hipEventRecord(e1, bi.stream));
-        HIP_OK(hipEventSynchronize(e1));
-        float ms = 0.0f;
-        HIP_OK(hipEventElapsedTime(&ms, e0, e1));
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        bi.opt.threads_per_instance = sh[0];
+        bi.opt.force_tile = sh[1];
+        const double ms = trial_ms(bi, nullptr, trial_replays);
         if (ms < best) {
             best = ms;
             best_opt = bi.opt;
@@ -1419,6 +1436,32 @@ static void autotune(BatchImpl& bi) {
     }
     bi.opt = best_opt;
     bi.opt.autotune = 0;
+    // size classes
+    std::vector<int> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return bi.prep[a].n_elems < bi.prep[b].n_elems; });
+    const int classes = n >= 8 ? 4 : 1;
+    bi.class_scale.assign(classes, cell_scale_of(bi.opt));
+    bi.class_elems.assign(classes, 0);
+    const double scales[] = {0.5, 1.0, 2.0};
+    for (int k = 0; k < classes; ++k) {
+        std::vector<uint8_t> only(n, 0);
+        const int lo = (int)((long long)n * k / classes), hi = (int)((long long)n * (k + 1) / classes);
+        for (int j = lo; j < hi; ++j) only[order[j]] = 1;
+        bi.class_elems[k] = bi.prep[order[hi - 1]].n_elems;
+        double best_k = INFINITY;
+        for (double cs : scales) {
+            for (int j = lo; j < hi; ++j) bi.host_tab[order[j]].cell_scale = cs;
+            put_table(bi);
+            const double ms = trial_ms(bi, &only, trial_replays);
+            if (ms < best_k) {
+                best_k = ms;
+                bi.class_scale[k] = cs;
+            }
+        }
+        for (int j = lo; j < hi; ++j) bi.host_tab[order[j]].cell_scale = bi.class_scale[k];
+    }
+    put_table(bi);
     reset_state(bi, true);
 }
 
@@ -1499,6 +1542,20 @@ int32_t crush_batch_options_get(const crush_batch* b, crush_batch_options* opt) 
     return guarded([&] {
         if (!b || !opt) fail(CRUSH_E_INPUT, "null argument");
         batch_options_of(*b->impl, opt);
+    });
+}
+
+int32_t crush_batch_class_scales(const crush_batch* b, int32_t cap, double* scale, int32_t* max_elems,
+                                 int32_t* n_classes) {
+    return guarded([&] {
+        if (!b || !n_classes || (cap > 0 && (!scale || !max_elems))) fail(CRUSH_E_INPUT, "null argument");
+        const BatchImpl& bi = *b->impl;
+        const int k = (int)bi.class_scale.size();
+        for (int i = 0; i < k && i < cap; ++i) {
+            scale[i] = bi.class_scale[i];
+            max_elems[i] = bi.class_elems[i];
+        }
+        *n_classes = k;
     });
 }
 

@@ -54,6 +54,11 @@ struct BatchImpl {
     std::vector<size_t> base_off;
     DevBuf<uint8_t> pool;
     DevBuf<InstDev> table;
+    std::vector<InstDev> host_tab;  // host copy of the table
+    // autotuned contact grid cell scale per size class (classes of equal
+    // count by element count; class_elems = the largest in each)
+    std::vector<double> class_scale;
+    std::vector<int> class_elems;
     // active lists: the instances (d_list) and the tiles of each tile
     // kernel (d_tiles[L_*]); their counts are device ints (d_counts[L_*])
     // read by the kernels
@@ -109,5 +114,7 @@ void dt_init(BatchImpl& bi);
 // Summaries of slots [0, n) into bi.host_summary.
 void collect(BatchImpl& bi, int n);
 int steps_per_graph(const crush_batch_options& o);
+// Contact grid cell scale of the options (1 when unset).
+double cell_scale_of(const crush_batch_options& o);
 
 }  // namespace crush

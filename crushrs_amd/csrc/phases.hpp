@@ -163,6 +163,10 @@ struct InstDev {
     // per group < 8: record slots of the accelerometer node (-1: none, or
     // a fixed node), the frame's origin and its x node (-1: no frame)
     const int32_t* g_probe;
+    // batched path: cell size of the instance's contact grids, as a
+    // multiple of the mean face extent (a speed knob: the grid never
+    // changes which face a node hits)
+    double cell_scale;
 };
 
 // Contact forces of 1-based step q (two buffers by the parity of q on the

@@ -169,7 +169,8 @@ static void upload(ServerImpl& S, Request& r) {
     BatchImpl& E = S.eng;
     uint8_t* base = E.pool.p + r.off;
     HIP_OK(hipMemcpyAsync(base, r.image.data(), r.image.size(), hipMemcpyHostToDevice, E.stream));
-    const InstDev inst = make_inst(r.P, r.o, reinterpret_cast<uintptr_t>(base));
+    InstDev inst = make_inst(r.P, r.o, reinterpret_cast<uintptr_t>(base));
+    inst.cell_scale = cell_scale_of(E.opt);
     HIP_OK(hipMemcpyAsync(E.table.p + r.slot, &inst, sizeof inst, hipMemcpyHostToDevice, E.stream));
     const uint8_t zero = 0;
     HIP_OK(hipMemcpyAsync(E.done_flag.p + r.slot, &zero, 1, hipMemcpyHostToDevice, E.stream));

@@ -349,6 +349,15 @@ int32_t crush_batch_run(crush_batch* b, crush_batch_stats* stats);
 /* Options in effect (after autotuning). */
 int32_t crush_batch_options_get(const crush_batch* b, crush_batch_options* opt);
 
+/* Contact grid cell scales the autotuner chose per size class (no
+ * counterpart in the reference, which fixes its own grouping:
+ * src/kernel_simd.rs:562).  The classes hold equal counts of instances by
+ * element count, ascending; up to cap of them are written to scale and
+ * max_elems (the largest element count in the class), and their number to
+ * n_classes (0 when the batch was not autotuned). */
+int32_t crush_batch_class_scales(const crush_batch* b, int32_t cap, double* scale, int32_t* max_elems,
+                                 int32_t* n_classes);
+
 /* Kernel timing on (1) or off (0): with it on, crush_batch_run launches
  * the steps directly instead of replaying the hipGraph, records HIP events
  * around each kernel and reports the summed kernel durations in its stats. */

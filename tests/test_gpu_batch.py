@@ -94,6 +94,24 @@ def test_batch_autotune_keeps_results():
         assert np.array_equal(a.displacement, b.displacement)
 
 
+def test_batch_size_class_autotune_keeps_results():
+    print("case: batch autotune with size classes (8 instances)", flush=True)
+    models = mix()[:3] + [ca.Model.crash(20.0 + 2.0 * k, 25.0) for k in range(5)]
+    ref = ca.Batch(models)
+    ref.run()
+    tuned = ca.Batch(models, autotune=1)
+    classes = tuned.class_scales()
+    assert len(classes) == 4 and all(cs in (0.5, 1.0, 2.0) for _, cs in classes)
+    assert [e for e, _ in classes] == sorted(e for e, _ in classes)
+    assert ref.class_scales() == []
+    tuned.run()
+    for i in range(len(models)):
+        a, b = ref.result(i), tuned.result(i)
+        assert a.steps == b.steps
+        assert np.array_equal(a.displacement, b.displacement)
+        assert np.array_equal(a.velocity, b.velocity)
+
+
 def test_batch_vs_oracle_delta_v():
     print("case: batch crash 30 30 vs oracle", flush=True)
     m = ca.Model.crash(30.0, 30.0)
