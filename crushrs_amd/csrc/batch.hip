@@ -161,6 +161,7 @@ ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     }
     o.n2e_ptr = b.put(P.n2e_ptr);
     o.n2e_idx = b.put(P.n2e_idx);
+
     // face-grid region of each contact: [starts: cells + 1][entries], plus a
     // cursor array of `cells` for grids built in device memory (contacts
     // with more faces than the LDS grid is sized for)
@@ -723,7 +724,7 @@ __global__ __launch_bounds__(T) void k_apply_tiles(const InstDev* __restrict__ t
 // configuration (the positions and erosion the next step starts from).
 template <int KIND, int T>
 __device__ __forceinline__ void force_tile(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
-                                           const int32_t* __restrict__ n_tiles, int chunk, double* s_w) {
+                                           const int32_t* __restrict__ n_tiles, int chunk, double* s_w, float* s_q) {
     const int b = xcd_tile(blockIdx.x, gridDim.x, chunk);
     if (b >= *n_tiles) return;
     const int2 tile = tiles[b];
@@ -733,7 +734,10 @@ __device__ __forceinline__ void force_tile(const InstDev* __restrict__ table, co
     const int end = KIND == CRUSH_MAT_HONEYCOMB ? D.n_hc : D.E;
     const int s = tile.y + (int)threadIdx.x;
     int er = 0;
-    if (s < end) er = force_element<KIND>(D, s);
+    // the elastic law parks the hourglass coordinates in LDS (see
+    // force_element_class); the honeycomb kernel keeps them in registers
+    constexpr int LS = KIND == CRUSH_MAT_HONEYCOMB ? 0 : T;
+    if (s < end) er = force_element<KIND, true, LS>(D, s, LS > 0 ? s_q + threadIdx.x : nullptr);
     const unsigned long long m = __ballot(er != 0);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(&c.eroded_step, (int)__popcll(m));
     if (c.adaptive && (c.step + 1) % c.chk == 0) dt_partial<T>(D, c.dt_floor, s, end, dt_slot<KIND, T>(D, tile.y), s_w);
@@ -749,16 +753,18 @@ template <int T>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(CRUSH_HC_WAVES, 8))) void k_force_hc(
     const InstDev* __restrict__ table, const int2* __restrict__ tiles, const int32_t* __restrict__ n_tiles, int chunk) {
     __shared__ double s_w[T / 64];
-    force_tile<CRUSH_MAT_HONEYCOMB, T>(table, tiles, n_tiles, chunk, s_w);
+    force_tile<CRUSH_MAT_HONEYCOMB, T>(table, tiles, n_tiles, chunk, s_w, nullptr);
 }
 
-// The elastic kernel at 4 waves per SIMD (128 VGPRs, some spills) runs 9 %
-// faster than at the 3 its 142 VGPRs would give.
+// The elastic kernel at 4 waves per SIMD: with the hourglass coordinates in
+// LDS through the Hencky law it fits 128 VGPRs without spills (8 % faster
+// than the build that spilled 21 VGPRs, which ran 9 % faster than 3 waves).
 template <int T>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_force_el(
     const InstDev* __restrict__ table, const int2* __restrict__ tiles, const int32_t* __restrict__ n_tiles, int chunk) {
     __shared__ double s_w[T / 64];
-    force_tile<CRUSH_MAT_ELASTIC, T>(table, tiles, n_tiles, chunk, s_w);
+    __shared__ float s_q[12 * T];
+    force_tile<CRUSH_MAT_ELASTIC, T>(table, tiles, n_tiles, chunk, s_w, s_q);
 }
 
 // Stable-step partials of instances that have not taken a step yet.

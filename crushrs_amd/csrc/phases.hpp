@@ -544,8 +544,8 @@ struct ClassConsts {
 // q = sum_i (u_i - u_0) gamma_i; (2) the material update gives tau;
 // (3) r = H_hg q; (4) each node's force tau F^-T grad_i V + r gamma_i^T is
 // written as it is formed.  Returns 1 when the element inverts now.
-template <int KIND, bool UNI>
-__device__ __forceinline__ int force_element_class(const InstDev& D, int s, int c) {
+template <int KIND, bool UNI, int LS>
+__device__ __forceinline__ int force_element_class(const InstDev& D, int s, int c, float* lq) {
 #pragma clang fp contract(off)
     const unsigned E = (unsigned)D.E, nh = (unsigned)D.n_hc;
     // corner forces: fe[(corner * E + slot) * 3 + axis], each corner's xyz
@@ -610,12 +610,22 @@ __device__ __forceinline__ int force_element_class(const InstDev& D, int s, int 
     float F[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) F[k] = hd[k] + ((k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f);
-    const float J = det3f(F);
+    float J = det3f(F);
     if (!(J > 0.0f)) {
         D.eroded[s] = 1;
 #pragma unroll
         for (int i = 0; i < 8; ++i) st_f32x3(rfe, 0.0f, 0.0f, 0.0f, vs3, i * E * 12u);
         return 1;
+    }
+    // with LS > 0 the hourglass coordinates wait in LDS (this thread's
+    // slots, stride LS) while the material update holds its temporaries, and
+    // F is formed again from the unchanged gradient afterwards: fewer values
+    // live through the update, the same arithmetic (the elastic kernels,
+    // whose Hencky law would otherwise spill; the honeycomb kernel fits
+    // 128 VGPRs without it and runs faster so)
+    if (LS > 0) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) lq[k * LS] = q[k];
     }
     const MatParams& mp = UNI ? *(const MatParams*)((cmat_ptr)D.mats + ((cint_ptr)D.cls_mat)[c])
                               : D.mats[D.cls_mat[c]];
@@ -626,6 +636,13 @@ __device__ __forceinline__ int force_element_class(const InstDev& D, int s, int 
         for (int k = 0; k < 25; ++k) st_f32(rst, st[k], vs, k * nh * 4u);
     } else {
         hencky_elastic(mp, hd, tau);
+    }
+    if (LS > 0) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) F[k] = hd[k] + ((k == 0 || k == 4 || k == 8) ? 1.0f : 0.0f);
+        J = det3f(F);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) q[k] = lq[k * LS];
     }
     // M = tau F^-T V
     float finv[9], M[9];
@@ -674,15 +691,17 @@ __device__ __forceinline__ int force_element_class(const InstDev& D, int s, int 
 // the class is taken as wave-uniform and its constants come in through
 // scalar loads.  A kernel that inlines this must need no SGPR spills (the
 // build checks): a build whose force kernel spilled SGPRs to VGPR lanes
-// gave results that changed from run to run.
-template <int KIND, bool SCALAR = true>
-__device__ __forceinline__ int force_element(const InstDev& D, int s) {
+// gave results that changed from run to run.  With LS > 0, lq is the
+// calling thread's first float of a 12 x LS workgroup array in LDS (slots
+// lq[k * LS]); with LS = 0 it is unused.
+template <int KIND, bool SCALAR, int LS>
+__device__ __forceinline__ int force_element(const InstDev& D, int s, float* lq) {
     const int c = D.ecls[s];
     if (SCALAR) {
         const int c0 = __builtin_amdgcn_readfirstlane(c);
-        if (__ballot(c != c0) == 0) return force_element_class<KIND, true>(D, s, c0);
+        if (__ballot(c != c0) == 0) return force_element_class<KIND, true, LS>(D, s, c0, lq);
     }
-    return force_element_class<KIND, false>(D, s, c);
+    return force_element_class<KIND, false, LS>(D, s, c, lq);
 }
 
 // Node-centric gather of the element forces into f (f64 sums in element

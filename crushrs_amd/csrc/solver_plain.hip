@@ -309,9 +309,11 @@ __global__ __launch_bounds__(256) void k_half(InstDev D) {
 // loads of the class constants (the same arithmetic as the batched tiles).
 template <int KIND>
 __global__ __launch_bounds__(256) void k_force(InstDev D, int begin, int end) {
+    constexpr int LS = KIND == CRUSH_MAT_HONEYCOMB ? 0 : 256;
+    __shared__ float s_q[12 * (LS > 0 ? LS : 1)];
     const int s = begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= end || D.ctl->done) return;
-    if (force_element<KIND, false>(D, s)) atomicAdd(&D.ctl->eroded_total, 1ULL);
+    if (force_element<KIND, false, LS>(D, s, s_q + threadIdx.x)) atomicAdd(&D.ctl->eroded_total, 1ULL);
 }
 
 __global__ __launch_bounds__(256) void k_gather_update(InstDev D) {
