@@ -211,7 +211,7 @@ ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
         off = (off + std::max<size_t>(n, 1) + 255) & ~size_t(255);
         return r;
     };
-    o.fe = scratch(sizeof(float) * 24 * (size_t)E);
+    o.fe = scratch(sizeof(float) * fe_floats(E));
     o.fgeo = scratch(sizeof(double) * FG * (size_t)P.n_cfaces);
     o.rec_f = scratch(sizeof(double) * 3 * (size_t)P.n_cnodes);
     o.fh_list = scratch(sizeof(int32_t) * FACE_HITS_CAP * (size_t)P.n_cfaces);

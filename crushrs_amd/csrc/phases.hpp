@@ -70,6 +70,16 @@ __device__ __forceinline__ void st_f32x3(rsrc_t r, float a, float b, float c, un
     __builtin_amdgcn_raw_buffer_store_b96(v, r, voff, soff, 0);
 }
 
+// Corner forces are stored by tiles of 64 element slots: each tile holds
+// its 8 corners' xyz for its 64 slots (corner-major inside the tile), so a
+// force pass wave stores each corner as one contiguous 768-byte run and the
+// 8 corner forces a node gathers lie within a few tiles.  fe_floats is the
+// array's size, fe_index the first float of (corner, slot).
+__host__ __device__ __forceinline__ size_t fe_floats(int E) { return (size_t)24 * (((size_t)E + 63) & ~size_t(63)); }
+__host__ __device__ __forceinline__ size_t fe_index(int corner, int slot) {
+    return ((size_t)((slot >> 6) * 8 + corner) * 64 + (slot & 63)) * 3;
+}
+
 // Everything a step touches for one model (one instance), as device
 // pointers.  The one-solve path fills it from its buffers; the batched path
 // keeps one per instance in a table.
@@ -548,13 +558,13 @@ template <int KIND, bool UNI, int LS>
 __device__ __forceinline__ int force_element_class(const InstDev& D, int s, int c, float* lq) {
 #pragma clang fp contract(off)
     const unsigned E = (unsigned)D.E, nh = (unsigned)D.n_hc;
-    // corner forces: fe[(corner * E + slot) * 3 + axis], each corner's xyz
+    // corner forces: fe[fe_index(corner, slot) + axis], each corner's xyz
     // in one 12-byte store (and one load in the gather)
-    const rsrc_t rfe = make_rsrc(D.fe, (size_t)24 * E * 4);
-    const unsigned vs = (unsigned)s * 4u, vs3 = (unsigned)s * 12u;
+    const rsrc_t rfe = make_rsrc(D.fe, fe_floats(D.E) * 4);
+    const unsigned vs = (unsigned)s * 4u, vs3 = (unsigned)fe_index(0, s) * 4u;
     if (D.eroded[s]) {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) st_f32x3(rfe, 0.0f, 0.0f, 0.0f, vs3, i * E * 12u);
+        for (int i = 0; i < 8; ++i) st_f32x3(rfe, 0.0f, 0.0f, 0.0f, vs3, i * 768u);
         return 0;
     }
     const rsrc_t rconn = make_rsrc(D.conn, (size_t)8 * E * 4);
@@ -614,7 +624,7 @@ __device__ __forceinline__ int force_element_class(const InstDev& D, int s, int 
     if (!(J > 0.0f)) {
         D.eroded[s] = 1;
 #pragma unroll
-        for (int i = 0; i < 8; ++i) st_f32x3(rfe, 0.0f, 0.0f, 0.0f, vs3, i * E * 12u);
+        for (int i = 0; i < 8; ++i) st_f32x3(rfe, 0.0f, 0.0f, 0.0f, vs3, i * 768u);
         return 1;
     }
     // with LS > 0 the hourglass coordinates wait in LDS (this thread's
@@ -681,7 +691,7 @@ __device__ __forceinline__ int force_element_class(const InstDev& D, int s, int 
 #pragma unroll
             for (int a = 0; a < 4; ++a) f[d] = fmf(rr[3 * a + d], ga[a], f[d]);
         }
-        st_f32x3(rfe, f[0], f[1], f[2], vs3, i * E * 12u);
+        st_f32x3(rfe, f[0], f[1], f[2], vs3, i * 768u);
     }
     return 0;
 }
@@ -709,7 +719,6 @@ __device__ __forceinline__ int force_element(const InstDev& D, int s, float* lq)
 __device__ __forceinline__ void gather_sum(const InstDev& D, int n, double* f) {
 #pragma clang fp contract(off)
     f[0] = f[1] = f[2] = 0.0;
-    const int E = D.E;
     const int kb = D.n2e_ptr[n], ke = D.n2e_ptr[n + 1];
     // up to 8 incidences (every node of a hex mesh without merged nodes):
     // all codes, then all element forces, are loaded before the sums so
@@ -723,7 +732,7 @@ __device__ __forceinline__ void gather_sum(const InstDev& D, int n, double* f) {
         // one 12-byte load per incidence (element 0 for a missing one).  A
         // plain struct load: this toolchain's raw_buffer_load_b96 builtin
         // keeps only the first dword when the elements are extracted.
-        const size_t base = code[j] >= 0 ? ((size_t)(code[j] & 7) * E + (code[j] >> 3)) * 3 : 0;
+        const size_t base = code[j] >= 0 ? fe_index(code[j] & 7, code[j] >> 3) : 0;
         const Float3 w = *reinterpret_cast<const Float3*>(D.fe + base);
         g[j][0] = code[j] >= 0 ? w.x : 0.0f;
         g[j][1] = code[j] >= 0 ? w.y : 0.0f;
@@ -736,7 +745,7 @@ __device__ __forceinline__ void gather_sum(const InstDev& D, int n, double* f) {
             for (int d = 0; d < 3; ++d) f[d] += (double)g[j][d];
     for (int k = kb + 8; k < ke; ++k) {
         const int cd = D.n2e_idx[k];
-        const Float3 w = *reinterpret_cast<const Float3*>(D.fe + ((size_t)(cd & 7) * E + (cd >> 3)) * 3);
+        const Float3 w = *reinterpret_cast<const Float3*>(D.fe + fe_index(cd & 7, cd >> 3));
         f[0] += (double)w.x;
         f[1] += (double)w.y;
         f[2] += (double)w.z;

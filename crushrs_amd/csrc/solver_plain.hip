@@ -59,7 +59,7 @@ struct DeviceModel {
         for (int k : {9, 13, 17})
             for (int i = 0; i < P.n_hc; ++i) st[(size_t)k * P.n_hc + i] = 1.0f;
         state.upload(st, s);
-        fe.alloc((size_t)24 * E);
+        fe.alloc(fe_floats(E));
         extra.alloc(E);
         eroded.alloc(E);
         eroded.zero(s);
