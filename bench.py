@@ -81,7 +81,7 @@ B_DT_ELEM, B_DT_NODE = 36, 24
 # Worker threads of one GPU's job on a shared host: the share the GPU box
 # gives one GPU (its scheduler runs one job per GPU with this many CPUs).
 CPU_SHARE = 16
-KERNELS = ("k_ctl", "k_grid", "k_search_tiles", "k_faceforce_tiles", "k_apply_tiles", "k_node_tiles",
+KERNELS = ("k_ctl", "k_grid", "k_search_tiles", "k_faceforce_tiles", "k_node_tiles",
            "k_force_hc", "k_force_el")
 
 

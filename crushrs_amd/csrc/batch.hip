@@ -2,7 +2,7 @@
 //
 // Each instance is one contiguous image in a device pool (its arrays plus a
 // control block) described by an InstDev pointer table.  One tick of every
-// active instance is eight launches (nine with a large contact), each over
+// active instance is seven launches (eight with a large contact), each over
 // all instances; a tick gathers one step and runs the contact, half step
 // and forces of the next:
 //   k_ctl              one workgroup per instance: end of the gathered step
@@ -14,11 +14,10 @@
 //                      stored to device memory;
 //   k_search_tiles     node search against the grid (tiles of contacting
 //                      nodes); k_faceforce_tiles face reactions (tiles of
-//                      faces); k_apply_tiles contact forces (tiles of
-//                      touched nodes);
+//                      faces);
 //   k_node_tiles       node-centric gather and second half step of the
-//                      step in flight, fused with the first half step of
-//                      the next (tiles of 256 nodes);
+//                      step in flight, fused with the contact force and
+//                      first half step of the next (tiles of 256 nodes);
 //   k_force_hc,        element forces and the next check step's stable-step
 //   k_force_el         partials (tiles of element slots), honeycomb and
 //                      elastic tiles in separate launches so each fills the
@@ -183,6 +182,7 @@ ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     o.tn_node = b.put(P.tn_node);
     o.tn_ptr = b.put(P.tn_ptr);
     o.tn_ent = b.put(P.tn_ent);
+    o.tn_of = b.put(P.tn_of);
     o.rec_face = b.put(std::vector<int32_t>(std::max(P.n_cnodes, 1), -1));
     o.fh_count = b.zeros(sizeof(int32_t) * std::max(P.n_cfaces, 1));
     o.g_nodes_ptr = b.put(P.g_nodes_ptr);
@@ -213,6 +213,7 @@ ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
     };
     o.fe = scratch(sizeof(float) * fe_floats(E));
     o.fgeo = scratch(sizeof(double) * FG * (size_t)P.n_cfaces);
+    o.fbox = scratch(sizeof(float4) * 2 * (size_t)P.n_cfaces);
     o.rec_f = scratch(sizeof(double) * 3 * (size_t)P.n_cnodes);
     o.fh_list = scratch(sizeof(int32_t) * FACE_HITS_CAP * (size_t)P.n_cfaces);
     o.face_force = scratch(sizeof(double) * 3 * (size_t)P.n_cfaces);
@@ -275,9 +276,11 @@ InstDev make_inst(const Prepared& P, const ImageOffsets& o, uintptr_t base) {
     I.tn_node = at<int32_t>(base, o.tn_node);
     I.tn_ptr = at<int32_t>(base, o.tn_ptr);
     I.tn_ent = at<int32_t>(base, o.tn_ent);
+    I.tn_of = at<int32_t>(base, o.tn_of);
     I.sblk_contact = nullptr;
     I.sblk_start = nullptr;
     I.fgeo = at<double>(base, o.fgeo);
+    I.fbox = at<float4>(base, o.fbox);
     I.rec_face = at<int32_t>(base, o.rec_face);
     I.rec_f = at<double>(base, o.rec_f);
     I.fh_count = at<int32_t>(base, o.fh_count);
@@ -406,19 +409,39 @@ __device__ __forceinline__ void crush_partials(const InstDev& D, int ng, CrushPa
         if (g < ng) {
             p.s[g] = 0.0;
             if (fr[g] >= 0) p.s[g] += D.u[3 * (size_t)fr[g] + ax[g]];
-            for (int i = b[g] + t + B; i < e[g]; i += B) p.s[g] += D.u[3 * (size_t)D.g_front[i] + ax[g]];
+            p.s[g] = front_sum<B>(D, b[g] + t + B, e[g], ax[g], p.s[g]);
         }
 }
 
 template <int B>
-__device__ __forceinline__ void crush_finish(int ng, CrushPart& p, double* s_w, double* cr) {
+__device__ __forceinline__ void crush_finish(int ng, CrushPart& p, double* s_wn, double* cr) {
 #pragma clang fp contract(off)
+    const int op[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    block_reduce_n<B, 8>(p.s, op, ng, s_wn);
 #pragma unroll
     for (int g = 0; g < 8; ++g)
-        if (g < ng) {
-            const double sum = block_reduce<B>(p.s[g], 0, s_w);
-            cr[g] = p.cnt[g] == 0 ? 0.0 : fabs(p.ua[g] - sum / (double)p.cnt[g]);
-        }
+        if (g < ng) cr[g] = p.cnt[g] == 0 ? 0.0 : fabs(p.ua[g] - p.s[g] / (double)p.cnt[g]);
+}
+
+// The control block moves between device memory and LDS as 8-byte words,
+// one per thread.
+constexpr int CTL_WORDS = (int)(sizeof(StepCtl) / 8);
+static_assert(sizeof(StepCtl) % 8 == 0, "StepCtl is copied as 8-byte words");
+template <int B>
+__device__ __forceinline__ void ctl_load(const StepCtl* src, StepCtl* c) {
+    static_assert(CTL_WORDS <= B, "one word per thread");
+    if ((int)threadIdx.x < CTL_WORDS)
+        reinterpret_cast<unsigned long long*>(c)[threadIdx.x] =
+            reinterpret_cast<const unsigned long long*>(src)[threadIdx.x];
+    __syncthreads();
+}
+// After the block's last change to c.
+template <int B>
+__device__ __forceinline__ void ctl_store(StepCtl* dst, const StepCtl* c) {
+    __syncthreads();
+    if ((int)threadIdx.x < CTL_WORDS)
+        reinterpret_cast<unsigned long long*>(dst)[threadIdx.x] =
+            reinterpret_cast<const unsigned long long*>(c)[threadIdx.x];
 }
 
 // Control pass of a tick (src/solver.rs:287-352), one workgroup per active
@@ -449,6 +472,7 @@ __global__ __launch_bounds__(B) void k_ctl(const InstDev* __restrict__ table, co
                                            const int32_t* __restrict__ n_list, int force_tile,
                                            uint8_t* __restrict__ done_flag, int* __restrict__ n_done) {
     __shared__ double s_w[B / 64];
+    __shared__ double s_wn[8 * (B / 64)];
     __shared__ double s_a[8];
     __shared__ StepCtl s_ctl;
     if ((int)blockIdx.x >= *n_list) return;
@@ -456,8 +480,7 @@ __global__ __launch_bounds__(B) void k_ctl(const InstDev* __restrict__ table, co
     const InstDev& D = table[inst];
     StepCtl* c = &s_ctl;
     const int t = threadIdx.x;
-    if (t == 0) *c = *D.ctl;
-    __syncthreads();
+    ctl_load<B>(D.ctl, c);
     if (c->done) return;
     const int ph = c->phase;
     const int ng = D.n_groups < 8 ? D.n_groups : 8;
@@ -471,9 +494,21 @@ __global__ __launch_bounds__(B) void k_ctl(const InstDev* __restrict__ table, co
     if (c->adaptive) {
         const int T = force_tile;
         const int slots = (D.n_hc + T - 1) / T + (D.E - D.n_hc + T - 1) / T;
-        for (int j = t; j < slots; j += B) {
-            mn = fmin(mn, D.dtp_min[j]);
-            sm += D.dtp_sum[j];
+        // four slots' loads in flight together, combined in slot order
+        for (int j0 = t; j0 < slots; j0 += 4 * B) {
+            double a[4], m[4];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                const bool on = j0 + k * B < slots;
+                a[k] = on ? D.dtp_min[j0 + k * B] : INFINITY;
+                m[k] = on ? D.dtp_sum[j0 + k * B] : 0.0;
+            }
+#pragma unroll
+            for (int k = 0; k < 4; ++k)
+                if (j0 + k * B < slots) {
+                    mn = fmin(mn, a[k]);
+                    sm += m[k];
+                }
         }
     }
     // (a) the last tick's gather
@@ -513,9 +548,9 @@ __global__ __launch_bounds__(B) void k_ctl(const InstDev* __restrict__ table, co
         if (stop) {
             finish_instance<B>(D, c, s_w);
             __syncthreads();
+            if (t == 0) c->phase = 0;
+            ctl_store<B>(D.ctl, c);
             if (t == 0) {
-                c->phase = 0;
-                *D.ctl = *c;
                 done_flag[inst] = 1;
                 atomicAdd(n_done, 1);
             }
@@ -536,14 +571,12 @@ __global__ __launch_bounds__(B) void k_ctl(const InstDev* __restrict__ table, co
                           (c->sep_steps > 0 && D.n_groups >= 2 && had_pair && quiet >= c->sep_steps && q % c->chk == 0);
         if (hold) {
             __syncthreads();
-            if (t == 0) {
-                c->phase = PH_GATHER;
-                *D.ctl = *c;
-            }
+            if (t == 0) c->phase = PH_GATHER;
+            ctl_store<B>(D.ctl, c);
             return;
         }
         double cr[8];
-        crush_finish<B>(ng, cp, s_w, cr);
+        crush_finish<B>(ng, cp, s_wn, cr);
         __syncthreads();
         if (t == 0) {
             for (int g = 0; g < ng; ++g) c->crush_max[g] = fmax(c->crush_max[g], cr[g]);
@@ -565,8 +598,11 @@ __global__ __launch_bounds__(B) void k_ctl(const InstDev* __restrict__ table, co
     // whole block
     const bool check = c->adaptive && c->step % c->chk == 0;
     if (check) {
-        mn = block_reduce<B>(mn, 1, s_w);
-        sm = block_reduce<B>(sm, 0, s_w);
+        double v[2] = {mn, sm};
+        const int op[2] = {1, 0};
+        block_reduce_n<B, 2>(v, op, 2, s_wn);
+        mn = v[0];
+        sm = v[1];
     }
     if (t == 0) {
         c->scale_now = 0;
@@ -581,8 +617,8 @@ __global__ __launch_bounds__(B) void k_ctl(const InstDev* __restrict__ table, co
             }
         }
         c->phase = PH_HALF | (closed ? PH_GATHER : 0);
-        *D.ctl = *c;
     }
+    ctl_store<B>(D.ctl, c);
 }
 
 // Contact (src/contact.rs:92-160), in four or five launches over all open
@@ -595,7 +631,7 @@ __global__ __launch_bounds__(B) void k_ctl(const InstDev* __restrict__ table, co
 // never changes which face a node hits: the search keeps the lowest
 // accepting face index.
 template <int B>
-__global__ __launch_bounds__(B) void k_grid(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_grid(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
                                             const int32_t* __restrict__ n_tiles) {
     __shared__ double s_w[B / 64];
     __shared__ int s_tmp[B / 64];
@@ -641,12 +677,13 @@ __global__ __launch_bounds__(B) void k_grid(const InstDev* __restrict__ table, c
 
 // (1a) face geometry of large moving contacts, tiles of T faces spread over
 // the GPU (one workgroup per contact would run their f64 face pass on one
-// CU each); tile.y = contact << 20 | first face in the contact.  Each face
-// is checked against its contact's kept grid, and a face outside it marks
-// the grid stale for k_grid.
+// CU each), four lanes per face (one triangle of the candidate region
+// each: contact_face_quad); tile.y = contact << 20 | first face in the
+// contact.  Each face is checked against its contact's kept grid, and a
+// face outside it marks the grid stale for k_grid.
 template <int T>
-__global__ __launch_bounds__(T) void k_facegeo_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
-                                                     const int32_t* __restrict__ n_tiles, int chunk) {
+__global__ __launch_bounds__(4 * T) void k_facegeo_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
+                                                         const int32_t* __restrict__ n_tiles, int chunk) {
     const int b = xcd_tile(blockIdx.x, gridDim.x, chunk);
     if (b >= *n_tiles) return;
     const int2 tile = tiles[b];
@@ -654,10 +691,12 @@ __global__ __launch_bounds__(T) void k_facegeo_tiles(const InstDev* __restrict__
     if (!(D.ctl->phase & PH_HALF)) return;
     const int ci = tile.y >> 20;
     const ContactDesc cd = D.cdesc[ci];
-    const int fl = (tile.y & 0xFFFFF) + (int)threadIdx.x;
+    // the four lanes of a face leave together
+    const int fl = (tile.y & 0xFFFFF) + (int)(threadIdx.x >> 2);
     if (fl >= cd.n_faces) return;
     const int f = cd.face_begin + fl;
-    contact_face(D, f);
+    contact_face_quad(D, f, (int)(threadIdx.x & 3));
+    if (threadIdx.x & 3) return;
     const FaceGrid prev = D.cgp[ci];
     if (!prev.built || prev.overflow || D.cf_static[f]) return;
     if (!kept_box_holds(D, cd, f, prev.margin)) atomicOr(&D.cgp[ci].stale, 1);
@@ -703,19 +742,6 @@ __global__ __launch_bounds__(T) void k_faceforce_tiles(const InstDev* __restrict
     if (!(D.ctl->phase & PH_HALF)) return;
     const int f = tile.y + (int)threadIdx.x;
     contact_face_force_wave(D, f, f < D.F);
-}
-
-template <int T>
-__global__ __launch_bounds__(T) void k_apply_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
-                                                   const int32_t* __restrict__ n_tiles, int chunk) {
-    const int b = xcd_tile(blockIdx.x, gridDim.x, chunk);
-    if (b >= *n_tiles) return;
-    const int2 tile = tiles[b];
-    const InstDev& D = table[tile.x];
-    const StepCtl& c = *D.ctl;
-    if (!(c.phase & PH_HALF)) return;
-    const int i = tile.y + (int)threadIdx.x;
-    if (i < D.T) contact_apply_node(D, i, fext_step(D, c.step + 1));
 }
 
 // Element forces of the open steps: one workgroup per tile of T element
@@ -853,17 +879,21 @@ __global__ __launch_bounds__(T) void k_node_tiles(const InstDev* __restrict__ ta
         f[2] = fi.z;
     }
     if (c.scale_now) im = mass_node(D, n);
+    // the contact force of the next step, from this node's apply list (in
+    // contact order, as the one-solve path's apply pass forms it), kept for
+    // that step's gather
+    double fe[3] = {0.0, 0.0, 0.0};
+    if (fl & NODE_CONTACT) {
+        const Double3 e = contact_apply_sum(D, D.tn_of[n]);
+        st_d3(fext_step(D, c.step + 1), n, e);
+        fe[0] = e.x;
+        fe[1] = e.y;
+        fe[2] = e.z;
+    }
     if (fx) {
         st_d3(D.u, n, Double3{0.0, 0.0, 0.0});
         st_d3(D.v, n, Double3{0.0, 0.0, 0.0});
         return;
-    }
-    double fe[3] = {0.0, 0.0, 0.0};
-    if (fl & NODE_CONTACT) {
-        const Double3 e = ld_d3(fext_step(D, c.step + 1), n);
-        fe[0] = e.x;
-        fe[1] = e.y;
-        fe[2] = e.z;
     }
     const Double3 u = ld_d3(D.u, n);
     const double ud[3] = {u.x, u.y, u.z};
@@ -935,7 +965,6 @@ static void add_tiles(const Prepared& P, int i, int T, std::vector<int2>* out) {
         }
         for (int q = 0; q < cd.n_nodes; q += CONTACT_TILE) out[L_SEARCH].push_back(make_int2(i, cd.node_begin + q));
     }
-    for (int k = 0; k < (int)P.tn_node.size(); k += CONTACT_TILE) out[L_APPLY].push_back(make_int2(i, k));
 }
 
 // Rebuild the active lists from the instances not yet retired.
@@ -1016,7 +1045,7 @@ static void launch_step(BatchImpl& bi, hipEvent_t* ev) {
                        T, bi.done_flag.p, bi.n_done.p);
     mark(K_GRID);
     if (bi.cap[L_FGEO] > 0)
-        hipLaunchKernelGGL(k_facegeo_tiles<CONTACT_TILE>, dim3(grid8(bi.cap[L_FGEO], bi.chunk[L_FGEO])), dim3(CONTACT_TILE), 0, st, tab,
+        hipLaunchKernelGGL(k_facegeo_tiles<CONTACT_TILE>, dim3(grid8(bi.cap[L_FGEO], bi.chunk[L_FGEO])), dim3(4 * CONTACT_TILE), 0, st, tab,
                            bi.d_tiles[L_FGEO].p, cnt + L_FGEO, bi.chunk[L_FGEO]);
     if (bi.cap[L_GRID] > 0) {
         const dim3 g(bi.cap[L_GRID]);
@@ -1032,21 +1061,18 @@ static void launch_step(BatchImpl& bi, hipEvent_t* ev) {
             hipLaunchKernelGGL(k_grid<1024>, g, dim3(1024), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID);
     }
     mark(K_SEARCH);
-    if (bi.cap[L_SEARCH] > 0)
+    if (bi.cap[L_SEARCH] > 0) {
         if (bi.search_lanes == 4)
             hipLaunchKernelGGL((k_search_tiles<CONTACT_TILE, 4>), dim3(grid8(bi.cap[L_SEARCH], bi.chunk[L_SEARCH])), dim3(4 * CONTACT_TILE), 0, st,
                                tab, bi.d_tiles[L_SEARCH].p, cnt + L_SEARCH, bi.chunk[L_SEARCH]);
         else
             hipLaunchKernelGGL((k_search_tiles<CONTACT_TILE, 1>), dim3(grid8(bi.cap[L_SEARCH], bi.chunk[L_SEARCH])), dim3(CONTACT_TILE), 0, st,
                                tab, bi.d_tiles[L_SEARCH].p, cnt + L_SEARCH, bi.chunk[L_SEARCH]);
+    }
     mark(K_FACEFORCE);
     if (bi.cap[L_FACE] > 0)
         hipLaunchKernelGGL(k_faceforce_tiles<CONTACT_TILE>, dim3(grid8(bi.cap[L_FACE], bi.chunk[L_FACE])), dim3(CONTACT_TILE), 0, st, tab,
                            bi.d_tiles[L_FACE].p, cnt + L_FACE, bi.chunk[L_FACE]);
-    mark(K_APPLY);
-    if (bi.cap[L_APPLY] > 0)
-        hipLaunchKernelGGL(k_apply_tiles<CONTACT_TILE>, dim3(grid8(bi.cap[L_APPLY], bi.chunk[L_APPLY])), dim3(CONTACT_TILE), 0, st, tab,
-                           bi.d_tiles[L_APPLY].p, cnt + L_APPLY, bi.chunk[L_APPLY]);
     mark(K_NODE);
     if (bi.cap[L_NODE] > 0)
         hipLaunchKernelGGL(k_node_tiles<NODE_TILE>, dim3(grid8(bi.cap[L_NODE], bi.chunk[L_NODE])), dim3(NODE_TILE), 0, st, tab,
@@ -1138,7 +1164,7 @@ void capture(BatchImpl& bi) {
     // XCD chunks of the force and node tiles: an eighth of an instance's
     // tiles (one instance spreads over the 8 L2s in 8 runs), at least 4
     // (many instances: a measured 2 % faster honeycomb pass than 1)
-    for (int L : {L_HC, L_EL, L_NODE, L_FGEO, L_SEARCH, L_FACE, L_APPLY}) {
+    for (int L : {L_HC, L_EL, L_NODE, L_FGEO, L_SEARCH, L_FACE}) {
         const int per_inst = (bi.cap[L] + std::max(bi.cap[L_INST], 1) - 1) / std::max(bi.cap[L_INST], 1);
         bi.chunk[L] = std::max(4, (per_inst + 7) / 8);
     }
@@ -1375,7 +1401,7 @@ void batch_run(BatchImpl& bi, crush_batch_stats* st) {
         st->latency_p99_ms = pct(0.99);
         const double* k = bi.t_kernel;
         st->k_end_ms = k[K_END];
-        st->k_contact_ms = k[K_GRID] + k[K_SEARCH] + k[K_FACEFORCE] + k[K_APPLY];
+        st->k_contact_ms = k[K_GRID] + k[K_SEARCH] + k[K_FACEFORCE];
         st->k_node_ms = k[K_NODE];
         st->k_force_hc_ms = k[K_FORCE_HC];
         st->k_force_el_ms = k[K_FORCE_EL];

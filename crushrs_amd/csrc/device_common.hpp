@@ -25,6 +25,23 @@ CRUSH_HD float rsqrt_f(float x) {
 #endif
 }
 
+// The largest float <= x and the smallest float >= x (NaN stays NaN): the
+// nearest float, stepped one ulp outward when it lies on the wrong side.
+// Boxes rounded so stay supersets of their f64 originals.
+CRUSH_HD float f32_next(float r, bool up) {
+    if (r == 0.0f) return up ? __builtin_bit_cast(float, 1u) : -__builtin_bit_cast(float, 1u);
+    const unsigned b = __builtin_bit_cast(unsigned, r);
+    return __builtin_bit_cast(float, (r > 0.0f) == up ? b + 1u : b - 1u);
+}
+CRUSH_HD float f32_down(double x) {
+    const float r = (float)x;
+    return (double)r > x ? f32_next(r, false) : r;
+}
+CRUSH_HD float f32_up(double x) {
+    const float r = (float)x;
+    return (double)r < x ? f32_next(r, true) : r;
+}
+
 // Per-solve control block in device memory.
 struct StepCtl {
     double t, dt, dt_cfl, end_time;
@@ -562,8 +579,14 @@ CRUSH_HD bool face_accepts(const double* __restrict__ fg, int F, int f, const do
 // triangle mapped onto the face plane along the triangle normal (an affine
 // map, so the corners suffice); the node itself lies up to max_distance
 // behind that.  Ill-conditioned faces fall back to the inflated AABB, which
-// the acceptance test applies first anyway.
-CRUSH_HD void face_geometry(const double (*x)[3], double max_d, double* out /* [FG_BLO] */) {
+// the acceptance test applies first anyway.  In three parts, so that the
+// four triangles can also be formed on four lanes: face_geometry_base (the
+// corners, normal, centre, inflated AABB and size; returns whether the
+// normal is usable), face_triangle_region (triangle t's part of the tight
+// region, min/max-accumulated into tlo/thi; returns false when the triangle
+// is degenerate or too oblique) and face_geometry_finish (the candidate
+// box).  Min and max do not depend on the order the triangles are taken in.
+CRUSH_HD bool face_geometry_base(const double (*x)[3], double max_d, double* out /* [FG_BLO] */) {
 #pragma clang fp contract(off)
     double e1[3], e2[3], n[3];
 #pragma unroll
@@ -592,47 +615,48 @@ CRUSH_HD void face_geometry(const double (*x)[3], double max_d, double* out /* [
         size = fmax(size, hi - lo);
     }
     out[FG_SZ] = size;
-    // tight candidate region
+    return __builtin_isfinite(nn) && nn > 0.0;
+}
+
+CRUSH_HD bool face_triangle_region(const double (*x)[3], const double* out, double max_d, int t, double* tlo,
+                                   double* thi) {
+#pragma clang fp contract(off)
     const int tri[4][3] = {{0, 1, 2}, {0, 2, 3}, {1, 2, 3}, {1, 3, 0}};
     const double cu[3] = {-C_TRI_TOL, 1.0 + 2.0 * C_TRI_TOL, -C_TRI_TOL};
     const double cv[3] = {-C_TRI_TOL, -C_TRI_TOL, 1.0 + 2.0 * C_TRI_TOL};
-    double tlo[3] = {INFINITY, INFINITY, INFINITY}, thi[3] = {-INFINITY, -INFINITY, -INFINITY};
-    bool valid = __builtin_isfinite(nn) && nn > 0.0;
-    for (int t = 0; t < 4 && valid; ++t) {
-        const double* a = x[tri[t][0]];
-        const double* b = x[tri[t][1]];
-        const double* c = x[tri[t][2]];
-        double v0[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
-        double v1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
-        double nt[3] = {v1[1] * v0[2] - v1[2] * v0[1], v1[2] * v0[0] - v1[0] * v0[2], v1[0] * v0[1] - v1[1] * v0[0]};
-        double ntn = sqrt(nt[0] * nt[0] + nt[1] * nt[1] + nt[2] * nt[2]);
-        double l0 = sqrt(v0[0] * v0[0] + v0[1] * v0[1] + v0[2] * v0[2]);
-        double l1 = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
-        if (!(ntn > 1e-6 * l0 * l1)) {
-            valid = false;
-            break;
-        }
-        double den = (n[0] * nt[0] + n[1] * nt[1] + n[2] * nt[2]) / ntn;
-        if (!(fabs(den) > 0.05)) {
-            valid = false;
-            break;
-        }
-        for (int k = 0; k < 3; ++k) {
-            double ep[3];
+    const double n[3] = {out[FG_N], out[FG_N + 1], out[FG_N + 2]};
+    const double* a = x[tri[t][0]];
+    const double* b = x[tri[t][1]];
+    const double* c = x[tri[t][2]];
+    double v0[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+    double v1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    double nt[3] = {v1[1] * v0[2] - v1[2] * v0[1], v1[2] * v0[0] - v1[0] * v0[2], v1[0] * v0[1] - v1[1] * v0[0]};
+    double ntn = sqrt(nt[0] * nt[0] + nt[1] * nt[1] + nt[2] * nt[2]);
+    double l0 = sqrt(v0[0] * v0[0] + v0[1] * v0[1] + v0[2] * v0[2]);
+    double l1 = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
+    if (!(ntn > 1e-6 * l0 * l1)) return false;
+    double den = (n[0] * nt[0] + n[1] * nt[1] + n[2] * nt[2]) / ntn;
+    if (!(fabs(den) > 0.05)) return false;
+    for (int k = 0; k < 3; ++k) {
+        double ep[3];
 #pragma unroll
-            for (int d = 0; d < 3; ++d) ep[d] = a[d] + cu[k] * v0[d] + cv[k] * v1[d];
-            double tt = (n[0] * (out[FG_C] - ep[0]) + n[1] * (out[FG_C + 1] - ep[1]) + n[2] * (out[FG_C + 2] - ep[2])) /
-                        (den * ntn);
+        for (int d = 0; d < 3; ++d) ep[d] = a[d] + cu[k] * v0[d] + cv[k] * v1[d];
+        double tt = (n[0] * (out[FG_C] - ep[0]) + n[1] * (out[FG_C + 1] - ep[1]) + n[2] * (out[FG_C + 2] - ep[2])) /
+                    (den * ntn);
 #pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                double q = ep[d] + tt * nt[d];
-                double q2 = q - max_d * n[d];
-                tlo[d] = fmin(tlo[d], fmin(q, q2));
-                thi[d] = fmax(thi[d], fmax(q, q2));
-            }
+        for (int d = 0; d < 3; ++d) {
+            double q = ep[d] + tt * nt[d];
+            double q2 = q - max_d * n[d];
+            tlo[d] = fmin(tlo[d], fmin(q, q2));
+            thi[d] = fmax(thi[d], fmax(q, q2));
         }
     }
-    const double margin = 1e-6 * size + 1e-12;
+    return true;
+}
+
+CRUSH_HD void face_geometry_finish(bool valid, const double* tlo, const double* thi, double* out) {
+#pragma clang fp contract(off)
+    const double margin = 1e-6 * out[FG_SZ] + 1e-12;
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
         double lo = out[FG_LO + d], hi = out[FG_HI + d];
@@ -643,6 +667,13 @@ CRUSH_HD void face_geometry(const double (*x)[3], double max_d, double* out /* [
         out[FG_CLO + d] = lo;
         out[FG_CHI + d] = hi;
     }
+}
+
+CRUSH_HD void face_geometry(const double (*x)[3], double max_d, double* out /* [FG_BLO] */) {
+    double tlo[3] = {INFINITY, INFINITY, INFINITY}, thi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    bool valid = face_geometry_base(x, max_d, out);
+    for (int t = 0; t < 4 && valid; ++t) valid = face_triangle_region(x, out, max_d, t, tlo, thi);
+    face_geometry_finish(valid, tlo, thi, out);
 }
 
 // V / A_max of a hexahedron from 8 positions (src/element.rs:176-186), f64.
@@ -663,7 +694,9 @@ CRUSH_HD void char_length_d(const double (*p)[3], double* length, double* volume
         double cx = u[1] * v[2] - u[2] * v[1], cy = u[2] * v[0] - u[0] * v[2], cz = u[0] * v[1] - u[1] * v[0];
         vol += (cx * w[0] + cy * w[1] + cz * w[2]) / 6.0;
     }
-    double amax = 0.0;
+    // the largest face area: x -> 0.5 sqrt(x) is non-decreasing, so the
+    // largest of the areas is 0.5 sqrt of the largest squared norm, exactly
+    double nmax = 0.0;
 #pragma unroll
     for (int f = 0; f < 6; ++f) {
         double a[3], b[3];
@@ -673,8 +706,9 @@ CRUSH_HD void char_length_d(const double (*p)[3], double* length, double* volume
             b[d] = p[faces[f][3]][d] - p[faces[f][1]][d];
         }
         double cx = a[1] * b[2] - a[2] * b[1], cy = a[2] * b[0] - a[0] * b[2], cz = a[0] * b[1] - a[1] * b[0];
-        amax = fmax(amax, 0.5 * sqrt(cx * cx + cy * cy + cz * cz));
+        nmax = fmax(nmax, cx * cx + cy * cy + cz * cz);
     }
+    const double amax = 0.5 * sqrt(nmax);
     *volume = vol;
     *length = (vol <= 0.0 || amax <= 0.0) ? INFINITY : vol / amax;
 }

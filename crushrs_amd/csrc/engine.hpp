@@ -34,18 +34,18 @@ constexpr double GRID_MARGIN = 0.1;
 struct ImageOffsets {
     size_t conn, ecls, cls_mat, cpk, cdtf, emass, mats, state, eroded, extra;
     size_t X, u, v, f_int, f_ext, f_ext2, inv_mass, mass, added, fixed, probe_node, g_probe, n2e_ptr, n2e_idx;
-    size_t cdesc, cn_node, cn_scale, cf_nodes, cf_contact, cf_static, cf_level, tn_node, tn_ptr, tn_ent, rec_face, fh_count;
+    size_t cdesc, cn_node, cn_scale, cf_nodes, cf_contact, cf_static, cf_level, tn_node, tn_ptr, tn_ent, tn_of, rec_face, fh_count;
     size_t g_nodes_ptr, g_nodes, g_front_ptr, g_front, g_accel, g_axis, g_mass, g_v0, g_dv, g_frame, ctl;
     size_t p_nodes_ptr, p_nodes;
     size_t init_bytes;
-    size_t fe, fgeo, rec_f, fh_list, face_force, cgrid, cgp, dtp_min, dtp_sum, pulse, hist, probe;
+    size_t fe, fgeo, fbox, rec_f, fh_list, face_force, cgrid, cgp, dtp_min, dtp_sum, pulse, hist, probe;
     size_t total;
 };
 
 // Active lists: the instance list and the tile lists of the tile kernels.
-enum { L_INST, L_HC, L_EL, L_NODE, L_FACE, L_GRID, L_SEARCH, L_APPLY, L_FGEO, N_LISTS };
+enum { L_INST, L_HC, L_EL, L_NODE, L_FACE, L_GRID, L_SEARCH, L_FGEO, N_LISTS };
 // Kernels of one tick, in launch order.
-enum { K_END, K_GRID, K_SEARCH, K_FACEFORCE, K_APPLY, K_NODE, K_FORCE_HC, K_FORCE_EL, N_KERNELS };
+enum { K_END, K_GRID, K_SEARCH, K_FACEFORCE, K_NODE, K_FORCE_HC, K_FORCE_EL, N_KERNELS };
 
 struct BatchImpl {
     crush_batch_options opt{};
@@ -68,7 +68,7 @@ struct BatchImpl {
     std::vector<uint8_t> host_done;
     int cnt[N_LISTS] = {};  // entries in the current lists
     int cap[N_LISTS] = {};  // grids of the captured graph
-    int chunk[N_LISTS] = {1, 1, 1, 1, 1, 1, 1, 1, 1};  // tiles per XCD chunk (force and node lists)
+    int chunk[N_LISTS] = {1, 1, 1, 1, 1, 1, 1, 1};  // tiles per XCD chunk (force and node lists)
     double headroom = 1.0;  // captured grids / current lists (server: > 1)
     bool need_capture = false;
     int grid_block = 64;    // k_grid workgroup: the largest moving contact's faces, in waves

@@ -123,9 +123,11 @@ struct InstDev {
     const int32_t* tn_node;
     const int32_t* tn_ptr;
     const int32_t* tn_ent;
+    const int32_t* tn_of;  // batched path: apply list of each node (-1 if none)
     const int32_t* sblk_contact;
     const int32_t* sblk_start;
     double* fgeo;
+    float4* fbox;  // [F][2]: each face's candidate box widened to f32 (lo, hi; w unused)
     int32_t* rec_face;
     double* rec_f;
     int32_t* fh_count;
@@ -222,6 +224,16 @@ __device__ __forceinline__ double mass_node(const InstDev& D, int n) {
     return im;
 }
 
+// Face record f: the f64 geometry (FG_BLO doubles, SoA) and the candidate
+// box rounded outward to f32 in one 32-byte record, with which the search
+// rejects what the f64 box rejects and a little less.
+__device__ __forceinline__ void store_face(const InstDev& D, int f, const double* g) {
+#pragma unroll
+    for (int k = 0; k < FG_BLO; ++k) D.fgeo[(size_t)k * D.F + f] = g[k];
+    D.fbox[2 * (size_t)f] = make_float4(f32_down(g[FG_CLO]), f32_down(g[FG_CLO + 1]), f32_down(g[FG_CLO + 2]), 0.0f);
+    D.fbox[2 * (size_t)f + 1] = make_float4(f32_up(g[FG_CHI]), f32_up(g[FG_CHI + 1]), f32_up(g[FG_CHI + 2]), 0.0f);
+}
+
 // Geometry of contact face f for the current configuration.
 __device__ __forceinline__ void contact_face(const InstDev& D, int f) {
 #pragma clang fp contract(off)
@@ -235,8 +247,39 @@ __device__ __forceinline__ void contact_face(const InstDev& D, int f) {
     }
     double g[FG_BLO];
     face_geometry(x, D.cdesc[D.cf_contact[f]].max_distance, g);
+    store_face(D, f, g);
+}
+
+// contact_face on the four lanes q = 0..3 of a quad of consecutive lanes,
+// each forming one triangle's part of the candidate region; every lane of
+// the quad calls it for the same face, and lane 0 writes the record (the
+// same record as contact_face's).
+__device__ __forceinline__ void contact_face_quad(const InstDev& D, int f, int q) {
+#pragma clang fp contract(off)
+    if (D.cf_static[f] && D.ctl->step > 0) return;
+    double x[4][3];
 #pragma unroll
-    for (int k = 0; k < FG_BLO; ++k) D.fgeo[(size_t)k * D.F + f] = g[k];
+    for (int k = 0; k < 4; ++k) {
+        const int n = D.cf_nodes[(size_t)k * D.F + f];
+        pos3(D.X, D.u, n, x[k]);
+    }
+    const double md = D.cdesc[D.cf_contact[f]].max_distance;
+    double g[FG_BLO];
+    double tlo[3] = {INFINITY, INFINITY, INFINITY}, thi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    bool valid = face_geometry_base(x, md, g);
+    if (valid) valid = face_triangle_region(x, g, md, q, tlo, thi);
+#pragma unroll
+    for (int off = 1; off < 4; off <<= 1) {
+        valid = (__shfl_xor((int)valid, off, 64) != 0) && valid;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            tlo[d] = fmin(tlo[d], __shfl_xor(tlo[d], off, 64));
+            thi[d] = fmax(thi[d], __shfl_xor(thi[d], off, 64));
+        }
+    }
+    if (q != 0) return;
+    face_geometry_finish(valid, tlo, thi, g);
+    store_face(D, f, g);
 }
 
 // Grid over one contact's face candidate boxes (built in LDS).
@@ -294,7 +337,8 @@ __device__ __forceinline__ int contact_node(const InstDev& D, const ContactDesc&
         if (inside) {
             // candidates four at a time: their ids and candidate boxes are
             // loaded together, and a node outside a face's candidate box
-            // (a superset of what the face can accept) skips the test
+            // (a superset of what the face can accept; here its f32 record,
+            // rounded outward, so a superset of the f64 box) skips the test
             const int cell = (ci[2] * g.dim[1] + ci[1]) * g.dim[0] + ci[0];
             const int qe = s_start[cell + 1];
             for (int q0 = s_start[cell] + 4 * sub; q0 < qe; q0 += 4 * L) {
@@ -305,12 +349,9 @@ __device__ __forceinline__ int contact_node(const InstDev& D, const ContactDesc&
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const int gf = fb + (fs[j] >= 0 ? fs[j] : 0);
-                    bool ok = fs[j] >= 0;
-#pragma unroll
-                    for (int d = 0; d < 3; ++d)
-                        ok = ok && p[d] >= D.fgeo[(size_t)(FG_CLO + d) * F + gf] &&
-                             p[d] <= D.fgeo[(size_t)(FG_CHI + d) * F + gf];
-                    in_box[j] = ok;
+                    const float4 lo = D.fbox[2 * (size_t)gf], hi = D.fbox[2 * (size_t)gf + 1];
+                    in_box[j] = fs[j] >= 0 && p[0] >= (double)lo.x && p[1] >= (double)lo.y && p[2] >= (double)lo.z &&
+                                p[0] <= (double)hi.x && p[1] <= (double)hi.y && p[2] <= (double)hi.z;
                 }
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
@@ -434,11 +475,9 @@ __device__ __forceinline__ void contact_face_force_wave(const InstDev& D, int f,
     }
 }
 
-// External (contact) force of touched node i, in contact order, into fext.
-// With FUSED the reactions of the node's faces are formed here (the same
-// arithmetic as contact_face_force) instead of read from face_force.
-template <bool FUSED = false>
-__device__ __forceinline__ void contact_apply_node(const InstDev& D, int i, double* fext) {
+// External (contact) force of touched node i, in contact order: its own
+// penalty records, less the reactions of the faces that contain it.
+__device__ __forceinline__ Double3 contact_apply_sum(const InstDev& D, int i) {
 #pragma clang fp contract(off)
     double s[3] = {0.0, 0.0, 0.0};
     for (int k = D.tn_ptr[i]; k < D.tn_ptr[i + 1]; ++k) {
@@ -449,19 +488,16 @@ __device__ __forceinline__ void contact_apply_node(const InstDev& D, int i, doub
                 for (int d = 0; d < 3; ++d) s[d] += D.rec_f[3 * (size_t)e + d];
         } else {
             const int f = -e - 1;
-            double r[3];
-            if (FUSED) {
-                face_reaction(D, f, r);
-            } else {
 #pragma unroll
-                for (int d = 0; d < 3; ++d) r[d] = D.face_force[3 * (size_t)f + d];
-            }
-#pragma unroll
-            for (int d = 0; d < 3; ++d) s[d] -= r[d];
+            for (int d = 0; d < 3; ++d) s[d] -= D.face_force[3 * (size_t)f + d];
         }
     }
-    const int n = D.tn_node[i];
-    st_d3(fext, n, Double3{s[0], s[1], s[2]});
+    return Double3{s[0], s[1], s[2]};
+}
+
+// contact_apply_sum of touched node i into fext.
+__device__ __forceinline__ void contact_apply_node(const InstDev& D, int i, double* fext) {
+    st_d3(fext, D.tn_node[i], contact_apply_sum(D, i));
 }
 
 // First half step of a free node (src/solver.rs:313-323): v_1/2 = v + dt/2
@@ -821,6 +857,51 @@ __device__ __forceinline__ double block_reduce(double v, int op, double* s_w) {
     return r;
 }
 
+// block_reduce of the first n <= K values of v together (value k with
+// op[k]), each in block_reduce's order, with one pair of barriers for all.
+// s_w holds K * B/64 doubles.
+template <int B, int K>
+__device__ __forceinline__ void block_reduce_n(double* v, const int* op, int n, double* s_w) {
+    constexpr int W = B / 64;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (k < n) v[k] = wave_reduce<B>(v[k], op[k]);
+    if (lane == 0)
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (k < n) s_w[k * W + wave] = v[k];
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+        if (k < n) {
+            double r = s_w[k * W];
+            for (int w = 1; w < W; ++w)
+                r = op[k] == 0 ? r + s_w[k * W + w] : (op[k] == 1 ? fmin(r, s_w[k * W + w]) : fmax(r, s_w[k * W + w]));
+            v[k] = r;
+        }
+    __syncthreads();
+}
+
+// s plus u[3 g_front[i] + ax] for i = i0, i0 + B, ... < e, added in that
+// order, with four terms' loads in flight together.
+template <int B>
+__device__ __forceinline__ double front_sum(const InstDev& D, int i0, int e, int ax, double s) {
+#pragma clang fp contract(off)
+    for (int i = i0; i < e; i += 4 * B) {
+        int idx[4];
+        double val[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) idx[j] = i + j * B < e ? D.g_front[i + j * B] : -1;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) val[j] = idx[j] >= 0 ? D.u[3 * (size_t)idx[j] + ax] : 0.0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+            if (idx[j] >= 0) s += val[j];
+    }
+    return s;
+}
+
 // Block-wide exclusive scan of s[0..n) in place (B threads): runs of
 // consecutive entries per thread, shuffled scan across each wave, waves
 // in order through s_tmp (B/64 ints).
@@ -1031,8 +1112,7 @@ __device__ bool grid_still_valid(const InstDev& D, const ContactDesc& cd, double
 template <int B>
 __device__ double group_crush(const InstDev& D, int g, double* s_w) {
     const int b = D.g_front_ptr[g], e = D.g_front_ptr[g + 1], ax = D.g_axis[g];
-    double s = 0.0;
-    for (int i = b + (int)threadIdx.x; i < e; i += B) s += D.u[3 * (size_t)D.g_front[i] + ax];
+    double s = front_sum<B>(D, b + (int)threadIdx.x, e, ax, 0.0);
     s = block_reduce<B>(s, 0, s_w);
     if (D.g_accel[g] < 0 || e <= b) return 0.0;
     return fabs(D.u[3 * (size_t)D.g_accel[g] + ax] - s / (double)(e - b));

@@ -510,8 +510,10 @@ void prepare(const crush_model& m, Prepared& p) {
                     lists[p.cf_nodes[(size_t)k * p.n_cfaces + d.face_begin + f]].push_back(-(d.face_begin + f) - 1);
         }
         p.tn_ptr.push_back(0);
+        p.tn_of.assign(N, -1);
         for (int n = 0; n < N; ++n) {
             if (lists[n].empty()) continue;
+            p.tn_of[n] = (int32_t)p.tn_node.size();
             p.tn_node.push_back(n);
             p.fixed[n] |= NODE_CONTACT;
             p.tn_ent.insert(p.tn_ent.end(), lists[n].begin(), lists[n].end());

@@ -78,6 +78,7 @@ struct Prepared {
     std::vector<int32_t> cf_contact;  // [F]
     std::vector<uint8_t> cf_static;   // [F] 1: every node of the face is fixed
     std::vector<int32_t> tn_node, tn_ptr, tn_ent;  // contact apply lists
+    std::vector<int32_t> tn_of;                    // [N] apply list of each node, -1 if none
     std::vector<int32_t> sblk_contact, sblk_start;  // search blocks
     int n_cnodes = 0, n_cfaces = 0;
     // time step

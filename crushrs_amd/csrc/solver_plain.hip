@@ -36,6 +36,7 @@ struct DeviceModel {
     DevBuf<ContactDesc> cdesc;
     DevBuf<int32_t> cn_node, cn_contact, cf_nodes, cf_contact, tn_node, tn_ptr, tn_ent, sblk_contact, sblk_start;
     DevBuf<double> cn_scale, fgeo, rec_f, face_force;
+    DevBuf<float4> fbox;
     DevBuf<int32_t> rec_face, fh_count, fh_list;
     DevBuf<int32_t> g_nodes_ptr, g_nodes, g_front_ptr, g_front, g_accel, g_axis;
     DevBuf<double> g_mass, g_v0, g_dv, pulse, hist;
@@ -94,6 +95,7 @@ struct DeviceModel {
         sblk_contact.upload(P.sblk_contact, s);
         sblk_start.upload(P.sblk_start, s);
         fgeo.alloc((size_t)FG * P.n_cfaces);
+        fbox.alloc(2 * (size_t)P.n_cfaces);
         rec_f.alloc(3 * (size_t)P.n_cnodes);
         rec_face.alloc(P.n_cnodes);
         face_force.alloc(3 * (size_t)P.n_cfaces);
@@ -176,6 +178,7 @@ struct DeviceModel {
         I.sblk_contact = sblk_contact.p;
         I.sblk_start = sblk_start.p;
         I.fgeo = fgeo.p;
+        I.fbox = fbox.p;
         I.rec_face = rec_face.p;
         I.rec_f = rec_f.p;
         I.fh_count = fh_count.p;

@@ -326,13 +326,14 @@ typedef struct crush_batch_stats {
     double k_end_ms;              /* per-instance control kernel (step end,
                                      stable step)                            */
     double k_contact_ms;          /* contact: faces and grids, search, face
-                                     reactions, apply                        */
-    double k_node_ms;             /* node pass: gather and next half step     */
+                                     reactions                               */
+    double k_node_ms;             /* node pass: gather, contact force and next
+                                     half step                               */
     double k_force_hc_ms;         /* honeycomb element forces                 */
     double k_force_el_ms;         /* elastic element forces                   */
     double k_ms[12];              /* each kernel of a tick in launch order:
                                      control, faces + grid, search,
-                                     reactions, apply, node pass, honeycomb,
+                                     reactions, node pass, honeycomb,
                                      elastic (the rest 0)                    */
 } crush_batch_stats;
 

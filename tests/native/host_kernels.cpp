@@ -33,6 +33,25 @@ extern "C" void hk_honeycomb(const float* hd, float youngs, const float* knots, 
 
 extern "C" void hk_eigen(const float* a, float* vals, float* vecs) { sym_eigen3f(a, vals, vecs); }
 
+// The stable step's characteristic length and volume of n hexes (p: 24
+// doubles each).
+extern "C" void hk_char_length(const double* p, int n, double* length, double* volume) {
+    for (int e = 0; e < n; ++e) {
+        double q[8][3];
+        for (int i = 0; i < 8; ++i)
+            for (int d = 0; d < 3; ++d) q[i][d] = p[24 * (size_t)e + 3 * i + d];
+        char_length_d(q, length + e, volume + e);
+    }
+}
+
+// The search's outward f32 rounding of candidate boxes, over n values.
+extern "C" void hk_f32_round(const double* x, int n, float* down, float* up) {
+    for (int i = 0; i < n; ++i) {
+        down[i] = f32_down(x[i]);
+        up[i] = f32_up(x[i]);
+    }
+}
+
 // The batched/one-model force_element (crushrs_amd/csrc/phases.hpp) restated
 // over explicit per-element arrays, as an element hook for the oracle's
 // solve loop (oracle_set_element_hook): relative displacements cast to f32,

@@ -1,5 +1,7 @@
 """The GPU kernels' f32 element arithmetic (compiled for the host) against
 the f64 oracle, element by element."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -116,3 +118,63 @@ def test_product_element_math_inside_oracle_solve(case):
     for x, y in zip(r.groups, o.groups):
         assert np.all(np.abs(x.delta_v - y.delta_v) < 0.05), (x.delta_v, y.delta_v)
         assert abs(x.crush_max - y.crush_max) < 2e-3
+
+
+def test_candidate_box_f32_rounding_is_outward_and_tight():
+    """The search's f32 candidate boxes (f32_down/f32_up, device_common.hpp)
+    bracket every f64 bound and are the nearest floats on each side
+    (checked with numpy's nextafter), so the f32 box is a superset of the
+    f64 box and rejects nothing the f64 box accepts."""
+    L = hk.load()
+    rng = np.random.default_rng(3)
+    base = np.concatenate([rng.normal(0.0, 1.0, 4000), rng.normal(0.0, 1e3, 2000), rng.uniform(-1e-30, 1e-30, 500)])
+    exact = rng.normal(0.0, 5.0, 500).astype(np.float32).astype(np.float64)
+    near = np.concatenate([np.nextafter(exact, np.inf), np.nextafter(exact, -np.inf)])
+    special = np.array([0.0, -0.0, 3.4e38, -3.4e38, 1e39, -1e39, np.inf, -np.inf, np.nan, 1e-46, -1e-46])
+    x = np.ascontiguousarray(np.concatenate([base, exact, near, special]))
+    dn = np.zeros(x.size, np.float32)
+    up = np.zeros(x.size, np.float32)
+    L.hk_f32_round(x.ctypes.data_as(C.POINTER(C.c_double)), x.size, hk.fp(dn), hk.fp(up))
+    fin = np.isfinite(x)
+    assert np.isnan(dn[~fin & np.isnan(x)]).all() and np.isnan(up[~fin & np.isnan(x)]).all()
+    xf = x[fin]
+    d, u = dn[fin].astype(np.float64), up[fin].astype(np.float64)
+    assert (d <= xf).all() and (u >= xf).all()
+    # tight: the next float outward from each bound lies past x
+    with np.errstate(over="ignore"):
+        assert (np.nextafter(dn[fin], np.float32(np.inf)).astype(np.float64) > xf).all()
+        assert (np.nextafter(up[fin], np.float32(-np.inf)).astype(np.float64) < xf).all()
+    # floats map to themselves
+    ex = np.isin(x, exact)
+    assert (dn[ex] == x[ex]).all() and (up[ex] == x[ex]).all()
+    assert dn[x == np.inf][0] == np.inf and up[x == -np.inf][0] == -np.inf
+
+
+def test_char_length_bitwise_equals_oracle():
+    """The device's characteristic length and volume (char_length_d, which
+    takes one square root of the largest squared face norm) equal the
+    oracle's restatement of src/element.rs:176-186 (a square root per face)
+    bit for bit, on boxes, distorted, inverted and degenerate hexes."""
+    L, O = hk.load(), of.load()
+    rng = np.random.default_rng(11)
+    cube = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0], [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]], float)
+    hexes = []
+    for k in range(3000):
+        scale = rng.uniform(0.01, 0.5, size=3)
+        h = cube * scale + rng.normal(0.0, 0.05 * scale.min() * (k % 4), size=(8, 3)) + rng.normal(0.0, 5.0, size=3)
+        hexes.append(h)
+    inv = cube.copy()
+    inv[[4, 5, 6, 7], 2] = -1.0
+    hexes += [inv, np.zeros((8, 3)), np.repeat(cube[:1], 8, axis=0), cube * 1e-160]
+    P = np.ascontiguousarray(np.array(hexes).reshape(len(hexes), 24))
+    n = len(hexes)
+    lg, vg = np.zeros(n), np.zeros(n)
+    L.hk_char_length(P.ctypes.data_as(C.POINTER(C.c_double)), n, lg.ctypes.data_as(C.POINTER(C.c_double)),
+                     vg.ctypes.data_as(C.POINTER(C.c_double)))
+    lo, vo = np.zeros(n), np.zeros(n)
+    for e in range(n):
+        l, v = np.zeros(1), np.zeros(1)
+        O.oracle_char_length(of.ptr(P[e].copy()), of.ptr(l), of.ptr(v))
+        lo[e], vo[e] = l[0], v[0]
+    assert np.array_equal(vg.view(np.int64), vo.view(np.int64))
+    assert np.array_equal(lg.view(np.int64), lo.view(np.int64))
