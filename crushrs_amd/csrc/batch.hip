@@ -103,7 +103,10 @@ static std::vector<int32_t> probe_nodes(const Prepared& P) {
 }
 
 // Stable-step partial slots of one instance (enough for the smallest tile).
-static size_t dtp_slots(const Prepared& P) { return (size_t)(P.n_hc + 63) / 64 + (P.n_elems - P.n_hc + 63) / 64 + 2; }
+// Stable-step partials of an instance: one per 64 element slots,
+// honeycomb slots first, then elastic ones (dt_slots of them).
+__host__ __device__ __forceinline__ int dt_slots(int n_hc, int E) { return (n_hc + 63) / 64 + (E - n_hc + 63) / 64; }
+static size_t dtp_slots(const Prepared& P) { return (size_t)dt_slots(P.n_hc, P.n_elems) + 2; }
 
 // Host image of one instance (initialized part only).
 ImageOffsets build_image(const Prepared& P, std::vector<uint8_t>& out) {
@@ -340,23 +343,26 @@ __device__ void finish_instance(const InstDev& D, StepCtl* c, double* s_w) {
     }
 }
 
-// Slot of a force tile among the stable-step partials of its instance:
-// honeycomb tiles first, then elastic ones.
-template <int KIND, int T>
+// The partial of the 64 slots of its kind from `first` (first minus the
+// kind's first slot is a multiple of 64).
+template <int KIND>
 __device__ __forceinline__ int dt_slot(const InstDev& D, int first) {
-    return KIND == CRUSH_MAT_HONEYCOMB ? first / T : (D.n_hc + T - 1) / T + (first - D.n_hc) / T;
+    return KIND == CRUSH_MAT_HONEYCOMB ? first / 64 : (D.n_hc + 63) / 64 + (first - D.n_hc) / 64;
 }
 
-// Stable-step partial of one tile (src/kernel.rs:207-258): min of the
-// element steps and sum of the scaling masses, in a fixed order.  Every
-// thread of the block calls it.
-template <int T>
-__device__ __forceinline__ void dt_partial(const InstDev& D, double dt_floor, int s, int end, int slot, double* s_w) {
+// Stable-step partial of one wave's 64 slots [first, first + 64) of a kind
+// that ends at `end` (src/kernel.rs:207-258): the min of the element steps
+// and the sum of the scaling masses, formed by the wave alone (no barrier)
+// in a fixed order.  Every lane of the wave calls it with its slot s.  The
+// min sets the step; the sum is only reported (the largest added mass).
+template <int KIND>
+__device__ __forceinline__ void dt_partial(const InstDev& D, double dt_floor, int s, int first, int end) {
     double mn = INFINITY, ex = 0.0;
     if (s < end) dt_element(D, s, dt_floor, &mn, &ex);
-    mn = block_reduce<T>(mn, 1, s_w);
-    ex = block_reduce<T>(ex, 0, s_w);
-    if (threadIdx.x == 0) {
+    mn = wave_reduce<64>(mn, 1);
+    ex = wave_reduce<64>(ex, 0);
+    if ((threadIdx.x & 63) == 0 && first < end) {
+        const int slot = dt_slot<KIND>(D, first);
         D.dtp_min[slot] = mn;
         D.dtp_sum[slot] = ex;
     }
@@ -469,7 +475,7 @@ __device__ __forceinline__ void ctl_store(StepCtl* dst, const StepCtl* c) {
 // step.
 template <int B>
 __global__ __launch_bounds__(B) void k_ctl(const InstDev* __restrict__ table, const int32_t* __restrict__ list,
-                                           const int32_t* __restrict__ n_list, int force_tile,
+                                           const int32_t* __restrict__ n_list,
                                            uint8_t* __restrict__ done_flag, int* __restrict__ n_done) {
     __shared__ double s_w[B / 64];
     __shared__ double s_wn[8 * (B / 64)];
@@ -486,14 +492,13 @@ __global__ __launch_bounds__(B) void k_ctl(const InstDev* __restrict__ table, co
     const int ng = D.n_groups < 8 ? D.n_groups : 8;
     // loads that no decision below changes, issued together: the probe
     // samples (a), the crush inputs of the step in flight (b) and the force
-    // tiles' stable-step partials (c), reduced per thread in slot order
+    // waves' stable-step partials (c), reduced per thread in slot order
     if ((ph & PH_GATHER) && (ph & PH_HALF) && D.n_pulse > 0 && t < ng) s_a[t] = accel_local_x_probed(D, t);
     CrushPart cp;
     if (ph & PH_HALF) crush_partials<B>(D, ng, cp);
     double mn = INFINITY, sm = 0.0;
     if (c->adaptive) {
-        const int T = force_tile;
-        const int slots = (D.n_hc + T - 1) / T + (D.E - D.n_hc + T - 1) / T;
+        const int slots = dt_slots(D.n_hc, D.E);
         // four slots' loads in flight together, combined in slot order
         for (int j0 = t; j0 < slots; j0 += 4 * B) {
             double a[4], m[4];
@@ -746,11 +751,11 @@ __global__ __launch_bounds__(T) void k_faceforce_tiles(const InstDev* __restrict
 
 // Element forces of the open steps: one workgroup per tile of T element
 // slots of one instance and one material kind.  When the next step is a
-// check step, the tile's stable-step partial is formed at the same
+// check step, each wave's stable-step partial is formed at the same
 // configuration (the positions and erosion the next step starts from).
 template <int KIND, int T>
 __device__ __forceinline__ void force_tile(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
-                                           const int32_t* __restrict__ n_tiles, int chunk, double* s_w, float* s_q) {
+                                           const int32_t* __restrict__ n_tiles, int chunk, float* s_q) {
     const int b = xcd_tile(blockIdx.x, gridDim.x, chunk);
     if (b >= *n_tiles) return;
     const int2 tile = tiles[b];
@@ -766,7 +771,8 @@ __device__ __forceinline__ void force_tile(const InstDev* __restrict__ table, co
     if (s < end) er = force_element<KIND, true, LS>(D, s, LS > 0 ? s_q + threadIdx.x : nullptr);
     const unsigned long long m = __ballot(er != 0);
     if ((threadIdx.x & 63) == 0 && m) atomicAdd(&c.eroded_step, (int)__popcll(m));
-    if (c.adaptive && (c.step + 1) % c.chk == 0) dt_partial<T>(D, c.dt_floor, s, end, dt_slot<KIND, T>(D, tile.y), s_w);
+    if (c.adaptive && (c.step + 1) % c.chk == 0)
+        dt_partial<KIND>(D, c.dt_floor, s, tile.y + (int)(threadIdx.x & ~63u), end);
 }
 
 // Lower bound of the waves per SIMD the honeycomb kernel is compiled for:
@@ -778,8 +784,7 @@ __device__ __forceinline__ void force_tile(const InstDev* __restrict__ table, co
 template <int T>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(CRUSH_HC_WAVES, 8))) void k_force_hc(
     const InstDev* __restrict__ table, const int2* __restrict__ tiles, const int32_t* __restrict__ n_tiles, int chunk) {
-    __shared__ double s_w[T / 64];
-    force_tile<CRUSH_MAT_HONEYCOMB, T>(table, tiles, n_tiles, chunk, s_w, nullptr);
+    force_tile<CRUSH_MAT_HONEYCOMB, T>(table, tiles, n_tiles, chunk, nullptr);
 }
 
 // The elastic kernel at 4 waves per SIMD: with the hourglass coordinates in
@@ -788,23 +793,21 @@ __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(CRUSH_HC_WAVE
 template <int T>
 __global__ __launch_bounds__(T) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_force_el(
     const InstDev* __restrict__ table, const int2* __restrict__ tiles, const int32_t* __restrict__ n_tiles, int chunk) {
-    __shared__ double s_w[T / 64];
     __shared__ float s_q[12 * T];
-    force_tile<CRUSH_MAT_ELASTIC, T>(table, tiles, n_tiles, chunk, s_w, s_q);
+    force_tile<CRUSH_MAT_ELASTIC, T>(table, tiles, n_tiles, chunk, s_q);
 }
 
 // Stable-step partials of instances that have not taken a step yet.
 template <int KIND, int T>
 __global__ __launch_bounds__(T) void k_dt_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
                                                 const int32_t* __restrict__ n_tiles) {
-    __shared__ double s_w[T / 64];
     if ((int)blockIdx.x >= *n_tiles) return;
     const int2 tile = tiles[blockIdx.x];
     const InstDev& D = table[tile.x];
     const StepCtl& c = *D.ctl;
     if (c.done || c.phase != 0 || c.step != 0 || !c.adaptive) return;
     const int end = KIND == CRUSH_MAT_HONEYCOMB ? D.n_hc : D.E;
-    dt_partial<T>(D, c.dt_floor, tile.y + (int)threadIdx.x, end, dt_slot<KIND, T>(D, tile.y), s_w);
+    dt_partial<KIND>(D, c.dt_floor, tile.y + (int)threadIdx.x, tile.y + (int)(threadIdx.x & ~63u), end);
 }
 
 // Node pass of a tick, tiles of T nodes of one instance: the gather and
@@ -1042,7 +1045,7 @@ static void launch_step(BatchImpl& bi, hipEvent_t* ev) {
     };
     mark(K_END);
     hipLaunchKernelGGL(k_ctl<B>, dim3(std::max(bi.cap[L_INST], 1)), dim3(B), 0, st, tab, bi.d_list.p, cnt + L_INST,
-                       T, bi.done_flag.p, bi.n_done.p);
+                       bi.done_flag.p, bi.n_done.p);
     mark(K_GRID);
     if (bi.cap[L_FGEO] > 0)
         hipLaunchKernelGGL(k_facegeo_tiles<CONTACT_TILE>, dim3(grid8(bi.cap[L_FGEO], bi.chunk[L_FGEO])), dim3(4 * CONTACT_TILE), 0, st, tab,
