@@ -831,10 +831,10 @@ __global__ __launch_bounds__(T) void k_node_tiles(const InstDev* __restrict__ ta
     if (ph == 0) return;
     const int n = tile.y + (int)threadIdx.x;
     if (n >= D.N) return;
-    const uint8_t fl = D.fixed[n];
+    const uint8_t fl = gld(D.fixed + n);
     const bool fx = (fl & NODE_FIXED) != 0;
     const double dth = step_dt(c);
-    double im = D.inv_mass[n];
+    double im = gld(D.inv_mass + n);
     double f[3], vd[3];
     {
         const Double3 v = ld_d3(D.v, n);
@@ -887,7 +887,7 @@ __global__ __launch_bounds__(T) void k_node_tiles(const InstDev* __restrict__ ta
     // that step's gather
     double fe[3] = {0.0, 0.0, 0.0};
     if (fl & NODE_CONTACT) {
-        const Double3 e = contact_apply_sum(D, D.tn_of[n]);
+        const Double3 e = contact_apply_sum(D, gld(D.tn_of + n));
         st_d3(fext_step(D, c.step + 1), n, e);
         fe[0] = e.x;
         fe[1] = e.y;

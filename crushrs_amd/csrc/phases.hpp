@@ -46,13 +46,34 @@ struct Float3 {
 struct Double3 {
     double x, y, z;
 };
-// Nodal vector n of an interleaved xyz array as one 24-byte access.
+// Loads and stores through the global address space.  An instance's arrays
+// are device memory reached through pointers in its table, which the
+// compiler cannot place, so plain accesses become flat ones; a global
+// access counts only against the vector-memory counter, so its waits do
+// not also wait for scalar loads and LDS.
+#define CRUSH_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ T gld(const T* p) {
+    return *(const CRUSH_GLOBAL T*)p;
+}
+template <class T>
+__device__ __forceinline__ void gst(T* p, T v) {
+    *(CRUSH_GLOBAL T*)p = v;
+}
+// Nodal vector n of an interleaved xyz array as one 24-byte access (the
+// three loads and stores merge).
 __device__ __forceinline__ Double3 ld_d3(const double* a, int n) {
-    return *reinterpret_cast<const Double3*>(a + 3 * (size_t)n);
+    const double* p = a + 3 * (size_t)n;
+    return Double3{gld(p), gld(p + 1), gld(p + 2)};
 }
 __device__ __forceinline__ void st_d3(double* a, int n, const Double3& v) {
-    *reinterpret_cast<Double3*>(a + 3 * (size_t)n) = v;
+    double* p = a + 3 * (size_t)n;
+    gst(p, v.x);
+    gst(p + 1, v.y);
+    gst(p + 2, v.z);
 }
+// Three consecutive floats as one 12-byte access.
+__device__ __forceinline__ Float3 ld_f3(const float* p) { return Float3{gld(p), gld(p + 1), gld(p + 2)}; }
 // Current position X + u of node n.
 __device__ __forceinline__ void pos3(const double* X, const double* u, int n, double* x) {
 #pragma clang fp contract(off)
@@ -216,11 +237,12 @@ __device__ __forceinline__ void dt_element(const InstDev& D, int s, double dt_fl
 __device__ __forceinline__ double mass_node(const InstDev& D, int n) {
 #pragma clang fp contract(off)
     double a = 0.0;
-    for (int k = D.n2e_ptr[n]; k < D.n2e_ptr[n + 1]; ++k) a += D.extra[D.n2e_idx[k] >> 3] / 8.0;
-    D.added[n] = a;
-    const double m = D.mass[n] + a;
+    const int kb = gld(D.n2e_ptr + n), ke = gld(D.n2e_ptr + n + 1);
+    for (int k = kb; k < ke; ++k) a += gld(D.extra + (gld(D.n2e_idx + k) >> 3)) / 8.0;
+    gst(D.added + n, a);
+    const double m = gld(D.mass + n) + a;
     const double im = m > 0.0 ? 1.0 / m : 0.0;
-    D.inv_mass[n] = im;
+    gst(D.inv_mass + n, im);
     return im;
 }
 
@@ -480,16 +502,21 @@ __device__ __forceinline__ void contact_face_force_wave(const InstDev& D, int f,
 __device__ __forceinline__ Double3 contact_apply_sum(const InstDev& D, int i) {
 #pragma clang fp contract(off)
     double s[3] = {0.0, 0.0, 0.0};
-    for (int k = D.tn_ptr[i]; k < D.tn_ptr[i + 1]; ++k) {
-        const int e = D.tn_ent[k];
+    const int kb = gld(D.tn_ptr + i), ke = gld(D.tn_ptr + i + 1);
+    for (int k = kb; k < ke; ++k) {
+        const int e = gld(D.tn_ent + k);
         if (e >= 0) {
-            if (D.rec_face[e] >= 0)
-#pragma unroll
-                for (int d = 0; d < 3; ++d) s[d] += D.rec_f[3 * (size_t)e + d];
+            if (gld(D.rec_face + e) >= 0) {
+                const Double3 r = ld_d3(D.rec_f, e);
+                s[0] += r.x;
+                s[1] += r.y;
+                s[2] += r.z;
+            }
         } else {
-            const int f = -e - 1;
-#pragma unroll
-            for (int d = 0; d < 3; ++d) s[d] -= D.face_force[3 * (size_t)f + d];
+            const Double3 r = ld_d3(D.face_force, -e - 1);
+            s[0] -= r.x;
+            s[1] -= r.y;
+            s[2] -= r.z;
         }
     }
     return Double3{s[0], s[1], s[2]};
@@ -755,33 +782,37 @@ __device__ __forceinline__ int force_element(const InstDev& D, int s, float* lq)
 __device__ __forceinline__ void gather_sum(const InstDev& D, int n, double* f) {
 #pragma clang fp contract(off)
     f[0] = f[1] = f[2] = 0.0;
-    const int kb = D.n2e_ptr[n], ke = D.n2e_ptr[n + 1];
+    const int kb = gld(D.n2e_ptr + n), ke = gld(D.n2e_ptr + n + 1);
     // up to 8 incidences (every node of a hex mesh without merged nodes):
     // all codes, then all element forces, are loaded before the sums so
-    // that the loads overlap; the sums keep element order
+    // that the loads overlap, without branches: a missing incidence loads
+    // the node's last code (or the array's first) and adds +0.0, which
+    // leaves the sum unchanged (the sum starts at +0.0 and so is never
+    // -0.0); the sums keep element order
     float g[8][3];
     int code[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) code[j] = kb + j < ke ? D.n2e_idx[kb + j] : -1;
+    const int last = ke > 0 ? ke - 1 : 0;
 #pragma unroll
     for (int j = 0; j < 8; ++j) {
-        // one 12-byte load per incidence (element 0 for a missing one).  A
-        // plain struct load: this toolchain's raw_buffer_load_b96 builtin
-        // keeps only the first dword when the elements are extracted.
+        const int c = gld(D.n2e_idx + min(kb + j, last));
+        code[j] = kb + j < ke ? c : -1;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        // one 12-byte load per incidence (element 0 for a missing one)
         const size_t base = code[j] >= 0 ? fe_index(code[j] & 7, code[j] >> 3) : 0;
-        const Float3 w = *reinterpret_cast<const Float3*>(D.fe + base);
+        const Float3 w = ld_f3(D.fe + base);
         g[j][0] = code[j] >= 0 ? w.x : 0.0f;
         g[j][1] = code[j] >= 0 ? w.y : 0.0f;
         g[j][2] = code[j] >= 0 ? w.z : 0.0f;
     }
 #pragma unroll
     for (int j = 0; j < 8; ++j)
-        if (code[j] >= 0)
 #pragma unroll
-            for (int d = 0; d < 3; ++d) f[d] += (double)g[j][d];
+        for (int d = 0; d < 3; ++d) f[d] += (double)g[j][d];
     for (int k = kb + 8; k < ke; ++k) {
-        const int cd = D.n2e_idx[k];
-        const Float3 w = *reinterpret_cast<const Float3*>(D.fe + fe_index(cd & 7, cd >> 3));
+        const int cd = gld(D.n2e_idx + k);
+        const Float3 w = ld_f3(D.fe + fe_index(cd & 7, cd >> 3));
         f[0] += (double)w.x;
         f[1] += (double)w.y;
         f[2] += (double)w.z;
