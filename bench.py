@@ -75,8 +75,8 @@ LAYOUT_HC_ELEM, LAYOUT_HC_NODE = 333, 24
 # HBM bytes the honeycomb force kernel actually moves per element-step,
 # from rocprofv3 FETCH_SIZE and WRITE_SIZE counters on a batched sweep,
 # each calibrated against a 1 GiB device copy (FETCH_SIZE x 2.0,
-# WRITE_SIZE x 1.0): 257 B read, 200 B written.
-TRAFFIC_HC_READ, TRAFFIC_HC_WRITE = 257.0, 200.0
+# WRITE_SIZE x 1.0): 217.6 B read, 198.7 B written.
+TRAFFIC_HC_READ, TRAFFIC_HC_WRITE = 217.6, 198.7
 B_DT_ELEM, B_DT_NODE = 36, 24
 # Worker threads of one GPU's job on a shared host: the share the GPU box
 # gives one GPU (its scheduler runs one job per GPU with this many CPUs).
