@@ -631,18 +631,23 @@ CRUSH_HD bool face_triangle_region(const double (*x)[3], const double* out, doub
     double v0[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
     double v1[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
     double nt[3] = {v1[1] * v0[2] - v1[2] * v0[1], v1[2] * v0[0] - v1[0] * v0[2], v1[0] * v0[1] - v1[1] * v0[0]};
-    double ntn = sqrt(nt[0] * nt[0] + nt[1] * nt[1] + nt[2] * nt[2]);
-    double l0 = sqrt(v0[0] * v0[0] + v0[1] * v0[1] + v0[2] * v0[2]);
-    double l1 = sqrt(v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2]);
-    if (!(ntn > 1e-6 * l0 * l1)) return false;
-    double den = (n[0] * nt[0] + n[1] * nt[1] + n[2] * nt[2]) / ntn;
-    if (!(fabs(den) > 0.05)) return false;
+    // the triangle is degenerate when |nt| <= 1e-6 |v0| |v1|, and too oblique
+    // to the face when |n . nt| <= 0.05 |nt|: both tested on squares, so
+    // without square roots (a region only widens the search, and at the
+    // boundary either answer leaves the box a superset of what the face
+    // accepts)
+    const double ntn2 = nt[0] * nt[0] + nt[1] * nt[1] + nt[2] * nt[2];
+    const double l02 = v0[0] * v0[0] + v0[1] * v0[1] + v0[2] * v0[2];
+    const double l12 = v1[0] * v1[0] + v1[1] * v1[1] + v1[2] * v1[2];
+    if (!(ntn2 > 1e-12 * l02 * l12)) return false;
+    const double dot = n[0] * nt[0] + n[1] * nt[1] + n[2] * nt[2];
+    if (!(dot * dot > 0.0025 * ntn2)) return false;
     for (int k = 0; k < 3; ++k) {
         double ep[3];
 #pragma unroll
         for (int d = 0; d < 3; ++d) ep[d] = a[d] + cu[k] * v0[d] + cv[k] * v1[d];
-        double tt = (n[0] * (out[FG_C] - ep[0]) + n[1] * (out[FG_C + 1] - ep[1]) + n[2] * (out[FG_C + 2] - ep[2])) /
-                    (den * ntn);
+        // the corner's projection onto the face plane along nt: ep + tt nt
+        double tt = (n[0] * (out[FG_C] - ep[0]) + n[1] * (out[FG_C + 1] - ep[1]) + n[2] * (out[FG_C + 2] - ep[2])) / dot;
 #pragma unroll
         for (int d = 0; d < 3; ++d) {
             double q = ep[d] + tt * nt[d];

@@ -26,11 +26,14 @@ def load():
         lib.hk_hencky.argtypes = [f32p, C.c_float, C.c_float, f32p]
         lib.hk_honeycomb.argtypes = [f32p, C.c_float, f32p, C.c_float, f32p, f32p]
         lib.hk_eigen.argtypes = [f32p, f32p, f32p]
+        f64p = C.POINTER(C.c_double)
         lib.hk_f32_round.argtypes = [C.POINTER(C.c_double), C.c_int, f32p, f32p]
         lib.hk_f32_round.restype = None
         lib.hk_char_length.argtypes = [C.POINTER(C.c_double), C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         lib.hk_char_length.restype = None
-        f64p = C.POINTER(C.c_double)
+        ip = C.POINTER(C.c_int)
+        lib.hk_face_check.argtypes = [f64p, C.c_int, C.c_double, f64p, C.c_int, f64p, ip, ip, ip]
+        lib.hk_face_check.restype = None
         lib.hk_hook_create.argtypes = [C.c_int]
         lib.hk_hook_create.restype = C.c_void_p
         lib.hk_hook_free.argtypes = [C.c_void_p]

@@ -44,6 +44,34 @@ extern "C" void hk_char_length(const double* p, int n, double* length, double* v
     }
 }
 
+// Face geometry of n faces (x: 12 doubles each) into g (FG_BLO doubles
+// each), then for each of m points per face (p: 3 doubles each) whether the
+// acceptance test accepts it (acc) and whether it lies in the f64 candidate
+// box (in64) and in the f32 box record (in32).
+extern "C" void hk_face_check(const double* x, int n, double max_d, const double* p, int m, double* g, int* acc,
+                              int* in64, int* in32) {
+    for (int f = 0; f < n; ++f) {
+        double q[4][3];
+        for (int k = 0; k < 4; ++k)
+            for (int d = 0; d < 3; ++d) q[k][d] = x[12 * (size_t)f + 3 * k + d];
+        double* gf = g + (size_t)FG_BLO * f;
+        face_geometry(q, max_d, gf);
+        for (int j = 0; j < m; ++j) {
+            const double* pp = p + 3 * ((size_t)f * m + j);
+            double sd;
+            const size_t o = (size_t)f * m + j;
+            acc[o] = face_accepts(gf, 1, 0, pp, max_d, &sd) ? 1 : 0;
+            bool a = true, b = true;
+            for (int d = 0; d < 3; ++d) {
+                a = a && pp[d] >= gf[FG_CLO + d] && pp[d] <= gf[FG_CHI + d];
+                b = b && pp[d] >= (double)f32_down(gf[FG_CLO + d]) && pp[d] <= (double)f32_up(gf[FG_CHI + d]);
+            }
+            in64[o] = a ? 1 : 0;
+            in32[o] = b ? 1 : 0;
+        }
+    }
+}
+
 // The search's outward f32 rounding of candidate boxes, over n values.
 extern "C" void hk_f32_round(const double* x, int n, float* down, float* up) {
     for (int i = 0; i < n; ++i) {

@@ -178,3 +178,50 @@ def test_char_length_bitwise_equals_oracle():
         lo[e], vo[e] = l[0], v[0]
     assert np.array_equal(vg.view(np.int64), vo.view(np.int64))
     assert np.array_equal(lg.view(np.int64), lo.view(np.int64))
+
+
+def test_candidate_boxes_hold_every_accepted_point():
+    """The product's face geometry (face_geometry, the tight candidate
+    region with its margin) and the search's f32 box record contain every
+    point the acceptance test of src/contact.rs:116-128 (face_accepts)
+    accepts, on flat, warped, sheared and nearly degenerate faces, with
+    points sampled over the faces, near their edges and behind them."""
+    L = hk.load()
+    rng = np.random.default_rng(5)
+    n, m, max_d = 600, 400, 0.05
+    quad = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0]], float)
+    faces = []
+    for k in range(n):
+        size = rng.uniform(0.02, 0.4)
+        q = quad * size
+        q[:, 2] += rng.normal(0.0, 0.1 * size * (k % 3), size=4)      # warp
+        q[:, 0] += rng.normal(0.0, 0.2 * size * (k % 2), size=4)      # shear
+        if k % 50 == 0:
+            q[3] = q[2] + 1e-9 * size                                  # nearly degenerate
+        th = rng.uniform(0, 2 * np.pi, 3)
+        Rz = np.array([[np.cos(th[0]), -np.sin(th[0]), 0], [np.sin(th[0]), np.cos(th[0]), 0], [0, 0, 1]])
+        Rx = np.array([[1, 0, 0], [0, np.cos(th[1]), -np.sin(th[1])], [0, np.sin(th[1]), np.cos(th[1])]])
+        faces.append(q @ (Rz @ Rx).T + rng.normal(0.0, 3.0, 3))
+    X = np.ascontiguousarray(np.array(faces).reshape(n, 12))
+    # points: barycentric-like samples over the face (including past its
+    # edges by the 10 % tolerance) and offsets along and against the normal
+    P = np.zeros((n, m, 3))
+    for f in range(n):
+        q = np.array(faces[f])
+        w = rng.uniform(-0.3, 1.3, size=(m, 2))
+        base = (q[0] * (1 - w[:, :1]) * (1 - w[:, 1:]) + q[1] * w[:, :1] * (1 - w[:, 1:]) + q[2] * w[:, :1] * w[:, 1:]
+                + q[3] * (1 - w[:, :1]) * w[:, 1:])
+        nrm = np.cross(q[1] - q[0], q[2] - q[0])
+        nrm /= max(np.linalg.norm(nrm), 1e-300)
+        P[f] = base + nrm * rng.uniform(-1.5 * max_d, 0.5 * max_d, size=(m, 1))
+    P = np.ascontiguousarray(P.reshape(-1))
+    G = np.zeros(n * 31)
+    acc, in64, in32 = (np.zeros(n * m, np.int32) for _ in range(3))
+    ip = C.POINTER(C.c_int)
+    dp = C.POINTER(C.c_double)
+    L.hk_face_check(X.ctypes.data_as(dp), n, max_d, P.ctypes.data_as(dp), m, G.ctypes.data_as(dp),
+                    acc.ctypes.data_as(ip), in64.ctypes.data_as(ip), in32.ctypes.data_as(ip))
+    assert acc.sum() > 0.1 * n * m              # the sample exercises the test
+    assert np.all(in64[acc == 1] == 1)
+    assert np.all(in32[acc == 1] == 1)
+    assert np.all(in32[in64 == 1] == 1)         # the f32 record holds the f64 box
