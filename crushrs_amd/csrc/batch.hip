@@ -474,15 +474,15 @@ __device__ __forceinline__ void ctl_store(StepCtl* dst, const StepCtl* c) {
 // delta-v, crush and last history record) are those after the next half
 // step.
 template <int B>
-__global__ __launch_bounds__(B) void k_ctl(const InstDev* __restrict__ table, const int32_t* __restrict__ list,
-                                           const int32_t* __restrict__ n_list,
-                                           uint8_t* __restrict__ done_flag, int* __restrict__ n_done) {
+__device__ __forceinline__ void ctl_block(const InstDev* __restrict__ table, const int32_t* __restrict__ list,
+                          const int32_t* __restrict__ n_list, uint8_t* __restrict__ done_flag,
+                          int* __restrict__ n_done, int bid) {
     __shared__ double s_w[B / 64];
     __shared__ double s_wn[8 * (B / 64)];
     __shared__ double s_a[8];
     __shared__ StepCtl s_ctl;
-    if ((int)blockIdx.x >= *n_list) return;
-    const int inst = list[blockIdx.x];
+    if (bid >= *n_list) return;
+    const int inst = list[bid];
     const InstDev& D = table[inst];
     StepCtl* c = &s_ctl;
     const int t = threadIdx.x;
@@ -626,6 +626,13 @@ __global__ __launch_bounds__(B) void k_ctl(const InstDev* __restrict__ table, co
     ctl_store<B>(D.ctl, c);
 }
 
+template <int B>
+__global__ __launch_bounds__(B) void k_ctl(const InstDev* __restrict__ table, const int32_t* __restrict__ list,
+                                           const int32_t* __restrict__ n_list,
+                                           uint8_t* __restrict__ done_flag, int* __restrict__ n_done) {
+    ctl_block<B>(table, list, n_list, done_flag, n_done, (int)blockIdx.x);
+}
+
 // Contact (src/contact.rs:92-160), in four or five launches over all open
 // steps.  (1) geometry of the faces of one contact of one instance (done by
 // k_facegeo_tiles for large contacts), and its face grid in the contact's
@@ -687,13 +694,13 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4, 8))) void 
 // contact.  Each face is checked against its contact's kept grid, and a
 // face outside it marks the grid stale for k_grid.
 template <int T>
-__global__ __launch_bounds__(4 * T) void k_facegeo_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
-                                                         const int32_t* __restrict__ n_tiles, int chunk) {
-    const int b = xcd_tile(blockIdx.x, gridDim.x, chunk);
+__device__ __forceinline__ void facegeo_block(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
+                              const int32_t* __restrict__ n_tiles, int chunk, int bid, int grid, bool any_phase) {
+    const int b = xcd_tile(bid, grid, chunk);
     if (b >= *n_tiles) return;
     const int2 tile = tiles[b];
     const InstDev& D = table[tile.x];
-    if (!(D.ctl->phase & PH_HALF)) return;
+    if (any_phase ? D.ctl->done != 0 : !(D.ctl->phase & PH_HALF)) return;
     const int ci = tile.y >> 20;
     const ContactDesc cd = D.cdesc[ci];
     // the four lanes of a face leave together
@@ -705,6 +712,37 @@ __global__ __launch_bounds__(4 * T) void k_facegeo_tiles(const InstDev* __restri
     const FaceGrid prev = D.cgp[ci];
     if (!prev.built || prev.overflow || D.cf_static[f]) return;
     if (!kept_box_holds(D, cd, f, prev.margin)) atomicOr(&D.cgp[ci].stale, 1);
+}
+
+template <int T>
+__global__ __launch_bounds__(4 * T) void k_facegeo_tiles(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
+                                                         const int32_t* __restrict__ n_tiles, int chunk) {
+    facegeo_block<T>(table, tiles, n_tiles, chunk, (int)blockIdx.x, (int)gridDim.x, false);
+}
+
+// The control pass and the large contacts' face geometry in one launch,
+// when the control workgroup has the face tiles' 4T threads: the first
+// n_ctl blocks (a multiple of 8, so that the face tiles keep their XCD
+// chunks) run k_ctl's work, the rest k_facegeo_tiles'.  The two touch
+// disjoint data (the face pass reads the displacement the last node pass
+// left, which the control pass does not write), so they overlap.  The face
+// pass cannot see the phase the control pass is setting, so it runs for
+// every instance not yet done: on a tick that holds a step for a gather
+// alone it recomputes the same face records from the same displacement,
+// and at most raises a face's grid margin level once more, which changes
+// the grid's layout but never a hit (the search keeps the lowest
+// accepting face).
+template <int T>
+__global__ __launch_bounds__(4 * T) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_ctl_facegeo(const InstDev* __restrict__ table,
+                                                       const int32_t* __restrict__ list,
+                                                       const int32_t* __restrict__ n_list,
+                                                       uint8_t* __restrict__ done_flag, int* __restrict__ n_done,
+                                                       int n_ctl, const int2* __restrict__ tiles,
+                                                       const int32_t* __restrict__ n_tiles, int chunk) {
+    if ((int)blockIdx.x < n_ctl)
+        ctl_block<4 * T>(table, list, n_list, done_flag, n_done, (int)blockIdx.x);
+    else
+        facegeo_block<T>(table, tiles, n_tiles, chunk, (int)blockIdx.x - n_ctl, (int)gridDim.x - n_ctl, true);
 }
 
 // (2) node search, tiles of T contacting nodes of one contact of one
@@ -1044,10 +1082,20 @@ static void launch_step(BatchImpl& bi, hipEvent_t* ev) {
         if (ev) HIP_OK(hipEventRecord(ev[k], st));
     };
     mark(K_END);
-    hipLaunchKernelGGL(k_ctl<B>, dim3(std::max(bi.cap[L_INST], 1)), dim3(B), 0, st, tab, bi.d_list.p, cnt + L_INST,
-                       bi.done_flag.p, bi.n_done.p);
+    // with a control workgroup of the face tiles' size, the large contacts'
+    // face geometry shares the control pass's launch
+    const bool merged = B == 4 * CONTACT_TILE && bi.cap[L_FGEO] > 0;
+    if (merged) {
+        const int n_ctl = (std::max(bi.cap[L_INST], 1) + 7) / 8 * 8;
+        hipLaunchKernelGGL(k_ctl_facegeo<CONTACT_TILE>, dim3(n_ctl + grid8(bi.cap[L_FGEO], bi.chunk[L_FGEO])),
+                           dim3(4 * CONTACT_TILE), 0, st, tab, bi.d_list.p, cnt + L_INST, bi.done_flag.p, bi.n_done.p,
+                           n_ctl, bi.d_tiles[L_FGEO].p, cnt + L_FGEO, bi.chunk[L_FGEO]);
+    } else {
+        hipLaunchKernelGGL(k_ctl<B>, dim3(std::max(bi.cap[L_INST], 1)), dim3(B), 0, st, tab, bi.d_list.p,
+                           cnt + L_INST, bi.done_flag.p, bi.n_done.p);
+    }
     mark(K_GRID);
-    if (bi.cap[L_FGEO] > 0)
+    if (!merged && bi.cap[L_FGEO] > 0)
         hipLaunchKernelGGL(k_facegeo_tiles<CONTACT_TILE>, dim3(grid8(bi.cap[L_FGEO], bi.chunk[L_FGEO])), dim3(4 * CONTACT_TILE), 0, st, tab,
                            bi.d_tiles[L_FGEO].p, cnt + L_FGEO, bi.chunk[L_FGEO]);
     if (bi.cap[L_GRID] > 0) {
