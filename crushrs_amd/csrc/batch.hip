@@ -643,14 +643,14 @@ __global__ __launch_bounds__(B) void k_ctl(const InstDev* __restrict__ table, co
 // never changes which face a node hits: the search keeps the lowest
 // accepting face index.
 template <int B>
-__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_grid(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
-                                            const int32_t* __restrict__ n_tiles) {
+__device__ __forceinline__ void grid_block(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
+                                           const int32_t* __restrict__ n_tiles, int bid, bool any_phase) {
     __shared__ double s_w[B / 64];
     __shared__ int s_tmp[B / 64];
-    if ((int)blockIdx.x >= *n_tiles) return;
-    const int2 tile = tiles[blockIdx.x];
+    if (bid >= *n_tiles) return;
+    const int2 tile = tiles[bid];
     const InstDev& D = table[tile.x];
-    if (!(D.ctl->phase & PH_HALF)) return;
+    if (any_phase ? D.ctl->done != 0 : !(D.ctl->phase & PH_HALF)) return;
     const int ci = tile.y;
     const ContactDesc cd = D.cdesc[ci];
     // faces of fixed nodes never move
@@ -685,6 +685,12 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4, 8))) void 
                                s_tmp, GRID_MARGIN);
     }
     if (threadIdx.x == 0) D.cgp[ci] = g;
+}
+
+template <int B>
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_grid(const InstDev* __restrict__ table, const int2* __restrict__ tiles,
+                                            const int32_t* __restrict__ n_tiles) {
+    grid_block<B>(table, tiles, n_tiles, (int)blockIdx.x, false);
 }
 
 // (1a) face geometry of large moving contacts, tiles of T faces spread over
@@ -732,6 +738,25 @@ __global__ __launch_bounds__(4 * T) void k_facegeo_tiles(const InstDev* __restri
 // and at most raises a face's grid margin level once more, which changes
 // the grid's layout but never a hit (the search keeps the lowest
 // accepting face).
+// The control pass and the grid pass in one launch, for batches with no
+// contact of more than 256 moving faces (whose face tiles must mark their
+// grids stale before the grid pass): the first n_ctl blocks run k_ctl's
+// work, the rest k_grid's at the control workgroup's size (the face pass
+// and the build loop over faces, so any size serves).  As in k_ctl_facegeo
+// the grid pass runs for every instance not done, and on a tick that holds
+// a step it recomputes the same face records and at most rebuilds a grid,
+// which never changes a hit.
+template <int B>
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(B <= 128 ? 3 : 2, 8))) void k_ctl_grid(const InstDev* __restrict__ table, const int32_t* __restrict__ list,
+                                                const int32_t* __restrict__ n_list, uint8_t* __restrict__ done_flag,
+                                                int* __restrict__ n_done, int n_ctl, const int2* __restrict__ tiles,
+                                                const int32_t* __restrict__ n_tiles) {
+    if ((int)blockIdx.x < n_ctl)
+        ctl_block<B>(table, list, n_list, done_flag, n_done, (int)blockIdx.x);
+    else
+        grid_block<B>(table, tiles, n_tiles, (int)blockIdx.x - n_ctl, true);
+}
+
 template <int T>
 __global__ __launch_bounds__(4 * T) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_ctl_facegeo(const InstDev* __restrict__ table,
                                                        const int32_t* __restrict__ list,
@@ -1085,7 +1110,13 @@ static void launch_step(BatchImpl& bi, hipEvent_t* ev) {
     // with a control workgroup of the face tiles' size, the large contacts'
     // face geometry shares the control pass's launch
     const bool merged = B == 4 * CONTACT_TILE && bi.cap[L_FGEO] > 0;
-    if (merged) {
+    // with no large contact, the grid pass shares the control pass's launch
+    const bool merged_grid = bi.cap[L_FGEO] == 0 && bi.cap[L_GRID] > 0;
+    if (merged_grid) {
+        const int n_ctl = std::max(bi.cap[L_INST], 1);
+        hipLaunchKernelGGL(k_ctl_grid<B>, dim3(n_ctl + bi.cap[L_GRID]), dim3(B), 0, st, tab, bi.d_list.p, cnt + L_INST,
+                           bi.done_flag.p, bi.n_done.p, n_ctl, bi.d_tiles[L_GRID].p, cnt + L_GRID);
+    } else if (merged) {
         const int n_ctl = (std::max(bi.cap[L_INST], 1) + 7) / 8 * 8;
         hipLaunchKernelGGL(k_ctl_facegeo<CONTACT_TILE>, dim3(n_ctl + grid8(bi.cap[L_FGEO], bi.chunk[L_FGEO])),
                            dim3(4 * CONTACT_TILE), 0, st, tab, bi.d_list.p, cnt + L_INST, bi.done_flag.p, bi.n_done.p,
@@ -1095,10 +1126,10 @@ static void launch_step(BatchImpl& bi, hipEvent_t* ev) {
                            cnt + L_INST, bi.done_flag.p, bi.n_done.p);
     }
     mark(K_GRID);
-    if (!merged && bi.cap[L_FGEO] > 0)
+    if (!merged && !merged_grid && bi.cap[L_FGEO] > 0)
         hipLaunchKernelGGL(k_facegeo_tiles<CONTACT_TILE>, dim3(grid8(bi.cap[L_FGEO], bi.chunk[L_FGEO])), dim3(4 * CONTACT_TILE), 0, st, tab,
                            bi.d_tiles[L_FGEO].p, cnt + L_FGEO, bi.chunk[L_FGEO]);
-    if (bi.cap[L_GRID] > 0) {
+    if (!merged_grid && bi.cap[L_GRID] > 0) {
         const dim3 g(bi.cap[L_GRID]);
         if (bi.grid_block == 64)
             hipLaunchKernelGGL(k_grid<64>, g, dim3(64), 0, st, tab, bi.d_tiles[L_GRID].p, cnt + L_GRID);
