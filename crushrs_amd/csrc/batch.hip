@@ -747,7 +747,7 @@ __global__ __launch_bounds__(4 * T) void k_facegeo_tiles(const InstDev* __restri
 // a step it recomputes the same face records and at most rebuilds a grid,
 // which never changes a hit.
 template <int B>
-__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(B <= 128 ? 3 : 2, 8))) void k_ctl_grid(const InstDev* __restrict__ table, const int32_t* __restrict__ list,
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_ctl_grid(const InstDev* __restrict__ table, const int32_t* __restrict__ list,
                                                 const int32_t* __restrict__ n_list, uint8_t* __restrict__ done_flag,
                                                 int* __restrict__ n_done, int n_ctl, const int2* __restrict__ tiles,
                                                 const int32_t* __restrict__ n_tiles) {
@@ -1110,12 +1110,17 @@ static void launch_step(BatchImpl& bi, hipEvent_t* ev) {
     // with a control workgroup of the face tiles' size, the large contacts'
     // face geometry shares the control pass's launch
     const bool merged = B == 4 * CONTACT_TILE && bi.cap[L_FGEO] > 0;
-    // with no large contact, the grid pass shares the control pass's launch
-    const bool merged_grid = bi.cap[L_FGEO] == 0 && bi.cap[L_GRID] > 0;
+    // with no large contact and a control workgroup of at most 128 threads
+    // (3 waves per SIMD; at 256 the grid pass's share runs at 2 and half its
+    // threads idle, slower than the two launches), the grid pass shares the
+    // control pass's launch
+    const bool merged_grid = B <= 128 && bi.cap[L_FGEO] == 0 && bi.cap[L_GRID] > 0;
     if (merged_grid) {
-        const int n_ctl = std::max(bi.cap[L_INST], 1);
-        hipLaunchKernelGGL(k_ctl_grid<B>, dim3(n_ctl + bi.cap[L_GRID]), dim3(B), 0, st, tab, bi.d_list.p, cnt + L_INST,
-                           bi.done_flag.p, bi.n_done.p, n_ctl, bi.d_tiles[L_GRID].p, cnt + L_GRID);
+        if constexpr (B <= 128) {
+            const int n_ctl = std::max(bi.cap[L_INST], 1);
+            hipLaunchKernelGGL(k_ctl_grid<B>, dim3(n_ctl + bi.cap[L_GRID]), dim3(B), 0, st, tab, bi.d_list.p,
+                               cnt + L_INST, bi.done_flag.p, bi.n_done.p, n_ctl, bi.d_tiles[L_GRID].p, cnt + L_GRID);
+        }
     } else if (merged) {
         const int n_ctl = (std::max(bi.cap[L_INST], 1) + 7) / 8 * 8;
         hipLaunchKernelGGL(k_ctl_facegeo<CONTACT_TILE>, dim3(n_ctl + grid8(bi.cap[L_FGEO], bi.chunk[L_FGEO])),
