@@ -770,14 +770,17 @@ __global__ __launch_bounds__(4 * T) __attribute__((amdgpu_waves_per_eu(2, 8))) v
         facegeo_block<T>(table, tiles, n_tiles, chunk, (int)blockIdx.x - n_ctl, (int)gridDim.x - n_ctl, true);
 }
 
-// (2) node search, tiles of T contacting nodes of one contact of one
-// instance, L lanes per node; penetrating nodes of pair contacts are
-// counted per instance.
+// (2) node search, tiles of CONTACT_TILE contacting nodes of one contact of
+// one instance, L lanes per node, T nodes per workgroup (CONTACT_TILE / T
+// consecutive workgroups share a tile); penetrating nodes of pair contacts
+// are counted per instance.
 template <int T, int L>
 __global__ __launch_bounds__(T * L) void k_search_tiles(const InstDev* __restrict__ table,
                                                         const int2* __restrict__ tiles,
                                                         const int32_t* __restrict__ n_tiles, int chunk) {
-    const int b = xcd_tile(blockIdx.x, gridDim.x, chunk);
+    constexpr int P = CONTACT_TILE / T;
+    static_assert(P * T == CONTACT_TILE, "a tile splits into whole workgroups");
+    const int b = xcd_tile((int)blockIdx.x / P, (int)gridDim.x / P, chunk);
     if (b >= *n_tiles) return;
     const int2 tile = tiles[b];
     const InstDev& D = table[tile.x];
@@ -786,7 +789,7 @@ __global__ __launch_bounds__(T * L) void k_search_tiles(const InstDev* __restric
     int ci = 0;
     while (ci + 1 < D.n_contacts && D.cdesc[ci + 1].node_begin <= tile.y) ++ci;
     const ContactDesc cd = D.cdesc[ci];
-    const int q = tile.y - cd.node_begin + (int)threadIdx.x / L;
+    const int q = tile.y - cd.node_begin + ((int)blockIdx.x % P) * T + (int)threadIdx.x / L;
     int hit = 0;
     if (q < cd.n_nodes) {
         const FaceGrid g = D.cgp[ci];
@@ -1056,11 +1059,15 @@ void rebuild_lists(BatchImpl& bi) {
         bi.grid_block = gb;
         bi.need_capture = true;
     }
-    // few search tiles: 4 lanes per contacting node; a server switches at
-    // half the threshold going down and at the threshold going up
+    // few search tiles: 4 lanes per contacting node, fewer still: 16; a
+    // server moves to more lanes at half a threshold going down and to fewer
+    // at the threshold going up
     const int ns = (int)tiles[L_SEARCH].size();
-    int sl = ns < SEARCH_SPLIT_TILES ? 4 : 1;
-    if (bi.headroom > 1.0 && ns >= SEARCH_SPLIT_TILES / 2 && ns < SEARCH_SPLIT_TILES) sl = bi.search_lanes;
+    auto lanes_at = [&](int scale2) {  // thresholds x scale2 / 2
+        return 2 * ns < SEARCH_WIDE_TILES * scale2 ? 16 : 2 * ns < SEARCH_SPLIT_TILES * scale2 ? 4 : 1;
+    };
+    int sl = lanes_at(2);
+    if (bi.headroom > 1.0) sl = std::min(std::max(bi.search_lanes, lanes_at(1)), lanes_at(2));
     if (sl != bi.search_lanes) {
         bi.search_lanes = sl;
         bi.need_capture = true;
@@ -1149,7 +1156,11 @@ static void launch_step(BatchImpl& bi, hipEvent_t* ev) {
     }
     mark(K_SEARCH);
     if (bi.cap[L_SEARCH] > 0) {
-        if (bi.search_lanes == 4)
+        if (bi.search_lanes == 16)
+            hipLaunchKernelGGL((k_search_tiles<CONTACT_TILE / 4, 16>), dim3(4 * grid8(bi.cap[L_SEARCH], bi.chunk[L_SEARCH])),
+                               dim3(4 * CONTACT_TILE), 0, st, tab, bi.d_tiles[L_SEARCH].p, cnt + L_SEARCH,
+                               bi.chunk[L_SEARCH]);
+        else if (bi.search_lanes == 4)
             hipLaunchKernelGGL((k_search_tiles<CONTACT_TILE, 4>), dim3(grid8(bi.cap[L_SEARCH], bi.chunk[L_SEARCH])), dim3(4 * CONTACT_TILE), 0, st,
                                tab, bi.d_tiles[L_SEARCH].p, cnt + L_SEARCH, bi.chunk[L_SEARCH]);
         else

@@ -20,8 +20,13 @@ constexpr int BATCH_GRID_CELLS = 1024;
 constexpr int BATCH_GRID_ENTRIES = 4096;
 // Cells of a grid built in LDS (k_grid workgroups of 512 or more threads)
 constexpr int LDS_GRID_CELLS = 4096;
-// Below this many search tiles a batch searches with 4 lanes per node
+// Below this many search tiles a batch searches with 4 lanes per node, and
+// below SEARCH_WIDE_TILES with 16 (a tile's 64 nodes over 4 workgroups)
 constexpr int SEARCH_SPLIT_TILES = 2048;
+#ifndef CRUSH_SEARCH_WIDE
+#define CRUSH_SEARCH_WIDE 256
+#endif
+constexpr int SEARCH_WIDE_TILES = CRUSH_SEARCH_WIDE;
 constexpr int NODE_TILE = 256;
 constexpr int CONTACT_TILE = 64;
 // Contact grids are kept over steps: faces are binned grown by this
@@ -72,7 +77,7 @@ struct BatchImpl {
     double headroom = 1.0;  // captured grids / current lists (server: > 1)
     bool need_capture = false;
     int grid_block = 64;    // k_grid workgroup: the largest moving contact's faces, in waves
-    int search_lanes = 1;   // k_search_tiles lanes per contacting node (1 or 4)
+    int search_lanes = 1;   // k_search_tiles lanes per contacting node (1, 4 or 16)
     bool fresh = true;  // no step taken since creation or reset
     DevBuf<BatchSummary> summary;
     std::vector<BatchSummary> host_summary;
