@@ -370,13 +370,24 @@ CRUSH_HD void sym_eigen3(const T* a, T* vals, T* vecs) {
 
 CRUSH_HD void sym_eigen3f(const float* a, float* vals, float* vecs) { sym_eigen3<float>(a, vals, vecs); }
 
+// A 3x3 matrix passed and returned by value (in registers).
+template <class T>
+struct Mat3v {
+    T a[9];
+};
+
 // Large-strain Hencky strain eps = ln(F F^T)/2 from principal stretches in
 // f64, where the smallest eigenvalues of b keep their relative accuracy.
 // Kept out of line: rare, and its f64 temporaries would otherwise raise the
-// register budget of every caller.
+// register budget of every caller.  It takes and returns its matrices by
+// value: a caller array whose address went to an out-of-line call would
+// live in scratch memory on every path.
 template <class T>
-__host__ __device__ __attribute__((noinline)) inline void hencky_log_strain_large(const T* hd, T* eps) {
+__host__ __device__ __attribute__((noinline)) inline Mat3v<T> hencky_log_strain_large(const Mat3v<T> hv) {
 #pragma clang fp contract(off)
+    const T* hd = hv.a;
+    Mat3v<T> ev;
+    T* eps = ev.a;
     double bd[9], lam[3], vec[9];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
@@ -395,6 +406,7 @@ __host__ __device__ __attribute__((noinline)) inline void hencky_log_strain_larg
 #pragma unroll
         for (int j = 0; j < 3; ++j)
             eps[3 * i + j] = (T)(e3[0] * vec[i] * vec[j] + e3[1] * vec[3 + i] * vec[3 + j] + e3[2] * vec[6 + i] * vec[6 + j]);
+    return ev;
 }
 
 CRUSH_HD float fmt(float a, float b, float c) { return fmaf(a, b, c); }
@@ -453,7 +465,12 @@ CRUSH_HD void hencky_elastic_t(const MatParams& mp, const T* hd, T* tau) {
 #pragma unroll
             for (int j = 0; j < 3; ++j) eps[3 * i + j] = T(0.25) * (l[3 * i + j] + l[3 * j + i]);
     } else {
-        hencky_log_strain_large<T>(hd, eps);
+        Mat3v<T> hv;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) hv.a[k] = hd[k];
+        const Mat3v<T> ev = hencky_log_strain_large<T>(hv);
+#pragma unroll
+        for (int k = 0; k < 9; ++k) eps[k] = ev.a[k];
     }
     const T vol = eps[0] + eps[4] + eps[8];
     const T pressure = (T)mp.bulk * vol;
