@@ -771,11 +771,15 @@ __global__ __launch_bounds__(4 * T) __attribute__((amdgpu_waves_per_eu(2, 8))) v
 }
 
 // (2) node search, tiles of CONTACT_TILE contacting nodes of one contact of
-// one instance, L lanes per node, T nodes per workgroup (CONTACT_TILE / T
-// consecutive workgroups share a tile); penetrating nodes of pair contacts
-// are counted per instance.
+// one instance (tile.y = contact << 20 | first node in the contact), L lanes
+// per node, T nodes per workgroup (CONTACT_TILE / T consecutive workgroups
+// share a tile); penetrating nodes of pair contacts are counted per
+// instance.
+#ifndef CRUSH_SEARCH_WAVES
+#define CRUSH_SEARCH_WAVES 5
+#endif
 template <int T, int L>
-__global__ __launch_bounds__(T * L) void k_search_tiles(const InstDev* __restrict__ table,
+__global__ __launch_bounds__(T * L) __attribute__((amdgpu_waves_per_eu(CRUSH_SEARCH_WAVES, 8))) void k_search_tiles(const InstDev* __restrict__ table,
                                                         const int2* __restrict__ tiles,
                                                         const int32_t* __restrict__ n_tiles, int chunk) {
     constexpr int P = CONTACT_TILE / T;
@@ -786,10 +790,9 @@ __global__ __launch_bounds__(T * L) void k_search_tiles(const InstDev* __restric
     const InstDev& D = table[tile.x];
     StepCtl& c = *D.ctl;
     if (!(c.phase & PH_HALF)) return;
-    int ci = 0;
-    while (ci + 1 < D.n_contacts && D.cdesc[ci + 1].node_begin <= tile.y) ++ci;
+    const int ci = tile.y >> 20;
     const ContactDesc cd = D.cdesc[ci];
-    const int q = tile.y - cd.node_begin + ((int)blockIdx.x % P) * T + (int)threadIdx.x / L;
+    const int q = (tile.y & 0xFFFFF) + ((int)blockIdx.x % P) * T + (int)threadIdx.x / L;
     int hit = 0;
     if (q < cd.n_nodes) {
         const FaceGrid g = D.cgp[ci];
@@ -1032,7 +1035,10 @@ static void add_tiles(const Prepared& P, int i, int T, std::vector<int2>* out) {
                      cd.n_faces);
             for (int f = 0; f < cd.n_faces; f += CONTACT_TILE) out[L_FGEO].push_back(make_int2(i, (ci << 20) | f));
         }
-        for (int q = 0; q < cd.n_nodes; q += CONTACT_TILE) out[L_SEARCH].push_back(make_int2(i, cd.node_begin + q));
+        if (cd.n_nodes > 0 && (ci >= (1 << 11) || cd.n_nodes >= (1 << 20)))
+            fail(CRUSH_E_INPUT, "contact %d of %d nodes: beyond the search tiles' 2048 contacts / 2^20 nodes", ci,
+                 cd.n_nodes);
+        for (int q = 0; q < cd.n_nodes; q += CONTACT_TILE) out[L_SEARCH].push_back(make_int2(i, (ci << 20) | q));
     }
 }
 
