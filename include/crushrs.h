@@ -334,7 +334,10 @@ typedef struct crush_batch_stats {
     double k_ms[12];              /* each kernel of a tick in launch order:
                                      control, faces + grid, search,
                                      reactions, node pass, honeycomb,
-                                     elastic (the rest 0)                    */
+                                     elastic (the rest 0); a launch that
+                                     runs the control pass with the face
+                                     tiles or the grid pass counts as
+                                     control                                 */
 } crush_batch_stats;
 
 /* Fill opt with the defaults. */
