@@ -775,6 +775,8 @@ __global__ __launch_bounds__(4 * T) __attribute__((amdgpu_waves_per_eu(2, 8))) v
 // per node, T nodes per workgroup (CONTACT_TILE / T consecutive workgroups
 // share a tile); penetrating nodes of pair contacts are counted per
 // instance.
+// At 5 waves per SIMD the search fits 94 VGPRs without spills (left free
+// it takes 98, which allows 4; at 6 it spills 76).
 #ifndef CRUSH_SEARCH_WAVES
 #define CRUSH_SEARCH_WAVES 5
 #endif
